@@ -1,0 +1,426 @@
+// C ABI of libdagtm_gpu: context, data movement, and the host-side control
+// of one DA-GTM iteration and of the annealing loop.  Kernels live in
+// dg_fp64.hip, dg_mixed.hip and dg_mstep.hip.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "dg_internal.h"
+
+#ifndef DG_VERSION_STRING
+#define DG_VERSION_STRING "dagtm_gpu gfx950"
+#endif
+
+int dg_fail(dg_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int dg_hip_check(dg_ctx *ctx, hipError_t e, const char *what) {
+    if (ctx) ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return DG_EHIP;
+}
+
+namespace {
+
+template <typename T>
+int dalloc(dg_ctx *c, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return dg_fail(c, DG_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return DG_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void free_problem(dg_ctx *c) {
+    dfree(c->T64); dfree(c->tt); dfree(c->FI); dfree(c->Y64); dfree(c->yy);
+    dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm); dfree(c->G); dfree(c->RT);
+    dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
+    dfree(c->partRT); dfree(c->partG);
+    dg_mixed_free(c);
+    c->loaded = false;
+    c->have_estep = false;
+}
+
+int sync_scalars(dg_ctx *c) {
+    DG_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, kScalCount * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+int select_device(dg_ctx *c) {
+    DG_HIP(c, hipSetDevice(c->device));
+    return DG_OK;
+}
+
+// Quality of mpicomm.c:463-500: [N((D/2) log(beta/2pi) - log K) + w * sum] / N
+double quality(long N, int K, int D, double beta, double sum, double w) {
+    double q = N * ((D / 2.0) * log(beta / (2.0 * M_PI)) - log((double)K));
+    q += w * sum;
+    return q / (double)N;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dg_version(void) { return DG_VERSION_STRING; }
+
+const char *dg_last_error(const dg_ctx *ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int dg_create(const dg_config *cfg, dg_ctx **out) {
+    if (!out) return DG_EINVAL;
+    *out = nullptr;
+    dg_config def{0, DG_FP64, 0.0, 2, 2};
+    const dg_config &cf = cfg ? *cfg : def;
+    if (cf.precision != DG_FP64 && cf.precision != DG_MIXED) return DG_EINVAL;
+    if (cf.lambda < 0.0) return DG_EINVAL;
+    dg_ctx *c = new (std::nothrow) dg_ctx();
+    if (!c) return DG_ENOMEM;
+    c->cfg = cf;
+    if (c->cfg.y_terms < 2 || c->cfg.y_terms > 3) c->cfg.y_terms = 2;
+    if (c->cfg.r_terms < 1 || c->cfg.r_terms > 2) c->cfg.r_terms = 2;
+    c->device = cf.device;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0 || cf.device < 0 || cf.device >= ndev) {
+        delete c;
+        (void)hipGetLastError();
+        return DG_EHIP;
+    }
+    int rc = DG_OK;
+    if ((rc = select_device(c)) != DG_OK) { delete c; return rc; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return DG_EHIP;
+    }
+    for (auto &ev : c->ev)
+        if (hipEventCreate(&ev) != hipSuccess) { dg_destroy(c); return DG_EHIP; }
+    if (dalloc(c, &c->scal, kScalCount) != DG_OK ||
+        hipHostMalloc((void **)&c->h_scal, kScalCount * sizeof(double), hipHostMallocDefault) !=
+            hipSuccess) {
+        dg_destroy(c);
+        return DG_ENOMEM;
+    }
+    std::memset(c->h_scal, 0, kScalCount * sizeof(double));
+    *out = c;
+    return DG_OK;
+}
+
+int dg_destroy(dg_ctx *c) {
+    if (!c) return DG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_problem(c);
+    dfree(c->scal);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DG_OK;
+}
+
+int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double *FI, int K,
+              int M1, long ldFI, const double *Y, long ldY, double beta) {
+    if (!c) return DG_EINVAL;
+    if (!T || !FI || !Y || N < 1 || D < 1 || K < 1 || M1 < 1 || ldT < D || ldFI < M1 || ldY < D ||
+        !(beta > 0.0))
+        return dg_fail(c, DG_EINVAL, "dg_upload: bad shape, stride or beta");
+    if (D > 192) return dg_fail(c, DG_EINVAL, "dg_upload: D > 192 is not supported");
+    if (M1 > 2048) return dg_fail(c, DG_EINVAL, "dg_upload: M+1 > 2048 is not supported");
+    int rc;
+    if ((rc = select_device(c)) != DG_OK) return rc;
+    free_problem(c);
+    c->N = N; c->D = D; c->K = K; c->M1 = M1; c->beta = beta;
+    c->ldt = ((D + 15) / 16) * 16;
+    const size_t ld = (size_t)c->ldt;
+    // N chunks of E-pass 2: enough (centre tile x chunk) workgroups to fill
+    // 256 CUs several times, each chunk at least one 64-point tile.
+    const long tiles = (N + 63) / 64;
+    const long ktiles = (K + 63) / 64;
+    long nc = (1024 + ktiles - 1) / ktiles;
+    if (nc > tiles) nc = tiles;
+    if (nc < 1) nc = 1;
+    c->NC = (int)nc;
+
+    if ((rc = dalloc(c, &c->T64, (size_t)N * ld)) || (rc = dalloc(c, &c->tt, N)) ||
+        (rc = dalloc(c, &c->FI, (size_t)K * M1)) || (rc = dalloc(c, &c->Y64, (size_t)K * ld)) ||
+        (rc = dalloc(c, &c->yy, K)) || (rc = dalloc(c, &c->W, (size_t)M1 * D)) ||
+        (rc = dalloc(c, &c->A, (size_t)M1 * M1)) || (rc = dalloc(c, &c->A0, (size_t)M1 * M1)) ||
+        (rc = dalloc(c, &c->Bm, (size_t)M1 * D)) || (rc = dalloc(c, &c->G, K)) ||
+        (rc = dalloc(c, &c->RT, (size_t)K * D)) || (rc = dalloc(c, &c->pm, N)) ||
+        (rc = dalloc(c, &c->pl, N)) || (rc = dalloc(c, &c->logg, N)) ||
+        (rc = dalloc(c, &c->logh, N)) || (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
+        (rc = dalloc(c, &c->partG, (size_t)c->NC * K))) {
+        free_problem(c);
+        return rc;
+    }
+    DG_HIP(c, hipMemsetAsync(c->T64, 0, (size_t)N * ld * sizeof(double), c->stream));
+    DG_HIP(c, hipMemsetAsync(c->Y64, 0, (size_t)K * ld * sizeof(double), c->stream));
+    DG_HIP(c, hipMemcpy2DAsync(c->T64, ld * sizeof(double), T, (size_t)ldT * sizeof(double),
+                               (size_t)D * sizeof(double), (size_t)N, hipMemcpyHostToDevice,
+                               c->stream));
+    DG_HIP(c, hipMemcpy2DAsync(c->FI, (size_t)M1 * sizeof(double), FI,
+                               (size_t)ldFI * sizeof(double), (size_t)M1 * sizeof(double),
+                               (size_t)K, hipMemcpyHostToDevice, c->stream));
+    DG_HIP(c, hipMemcpy2DAsync(c->Y64, ld * sizeof(double), Y, (size_t)ldY * sizeof(double),
+                               (size_t)D * sizeof(double), (size_t)K, hipMemcpyHostToDevice,
+                               c->stream));
+    if ((rc = dg_rownorms(c, c->T64, N, c->ldt, D, c->tt))) return rc;
+    if ((rc = dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy))) return rc;
+    // sum_n |t_n|^2 for the closed-form beta update
+    if ((rc = dg_sum_into(c, c->tt, N, kScalBetaSum)) || (rc = sync_scalars(c))) return rc;
+    c->sum_tt = c->h_scal[kScalBetaSum];
+    c->loaded = true;
+    if (c->cfg.precision == DG_MIXED) {
+        if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
+    }
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+int dg_set_state(dg_ctx *c, const double *Y, long ldY, double beta) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_set_state before dg_upload");
+    if (!Y || ldY < c->D || !(beta > 0.0)) return dg_fail(c, DG_EINVAL, "dg_set_state: bad args");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    DG_HIP(c, hipMemcpy2DAsync(c->Y64, (size_t)c->ldt * sizeof(double), Y,
+                               (size_t)ldY * sizeof(double), (size_t)c->D * sizeof(double),
+                               (size_t)c->K, hipMemcpyHostToDevice, c->stream));
+    if ((rc = dg_rownorms(c, c->Y64, c->K, c->ldt, c->D, c->yy))) return rc;
+    c->beta = beta;
+    c->have_estep = false;
+    if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+int dg_estep(dg_ctx *c, int isDA, double temp, dg_estep_out *out) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_estep before dg_upload");
+    if (isDA && !(temp > 0.0)) return dg_fail(c, DG_EINVAL, "dg_estep: temp must be > 0");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    if (c->cfg.precision == DG_MIXED) rc = dg_estep_mixed(c, isDA, temp);
+    else rc = dg_estep_fp64(c, isDA, temp);
+    if (rc) return rc;
+    if (c->timing) DG_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    if ((rc = sync_scalars(c))) return rc;
+    if (c->timing) {
+        float a = 0, b = 0, d = 0;
+        (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+        c->t_ms[0] = a; c->t_ms[1] = b; c->t_ms[2] = d;
+    }
+    c->have_estep = true;
+    c->last_isDA = isDA != 0;
+    c->last_temp = isDA ? temp : 1.0;
+    const double sg = c->h_scal[kScalSumLogG], sh = c->h_scal[kScalSumLogH];
+    if (out) {
+        out->llh = quality(c->N, c->K, c->D, c->beta, sg, 1.0);
+        out->nfree = isDA ? quality(c->N, c->K, c->D, c->beta, sh, temp) : -out->llh;
+        out->nzero = (long)c->h_scal[kScalNZero];
+    }
+    return DG_OK;
+}
+
+int dg_mstep(dg_ctx *c, double *beta_out) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded || !c->have_estep) return dg_fail(c, DG_ESTATE, "dg_mstep needs a dg_estep first");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    if (c->timing) DG_HIP(c, hipEventRecord(c->ev[4], c->stream));
+    if ((rc = dg_mstep_run(c))) return rc;
+    if (c->timing) DG_HIP(c, hipEventRecord(c->ev[5], c->stream));
+    if ((rc = sync_scalars(c))) return rc;
+    if (c->timing) {
+        float a = 0;
+        (void)hipEventElapsedTime(&a, c->ev[4], c->ev[5]);
+        c->t_ms[3] = a;
+    }
+    c->have_estep = false;
+    if (c->h_scal[kScalSolveOK] != 1.0)
+        return dg_fail(c, DG_ESOLVE, "M-step Cholesky failed after jitter retries");
+    const double s = c->sum_tt + c->h_scal[kScalBetaSum];
+    c->beta = ((double)c->N * c->D) / s;
+    if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
+    if (beta_out) *beta_out = c->beta;
+    return DG_OK;
+}
+
+int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_iter_cb cb,
+              void *user) {
+    if (!c || !p || !res) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_anneal before dg_upload");
+    const int QN = 50;   // INIT_QUEUE_SIZE, dagtm.cxx:45
+    double Q[QN];
+    int qnext = 0;
+    double temp = p->isDA ? p->start_temp : 1.0;
+    int isLast = 0, nloop = 0, ntemps = 0, exit_maxloop = 0;
+    double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff = 0;
+    int rc;
+    auto qmean = [&]() { double s = 0; for (int i = 0; i < QN; ++i) s += Q[i]; return s / QN; };
+    auto absv = [](double a) { return (a > 0) ? a : -a; };
+    do {
+        if (temp < 1.0) { temp = 1.0; isLast = 1; }
+        for (int i = 0; i < QN; ++i) Q[i] = DBL_MAX;
+        qnext = 0;
+        ntemps++;
+        int lloop = 0;
+        while (true) {
+            dg_estep_out eo;
+            if ((rc = dg_estep(c, p->isDA, temp, &eo))) return rc;
+            llh = eo.llh; nfree = eo.nfree;
+            qual_old = qual;
+            lloop++; nloop++;
+            qual = p->isDA ? nfree : llh;
+            diff = qual - qual_old;
+            if (cb) cb(user, nloop, lloop, temp, nfree, llh, diff, c->beta);
+            if (absv(diff) < p->eps) break;
+            if (nloop > QN) {
+                const double old = qmean();
+                Q[qnext] = qual; qnext = (qnext + 1) % QN;
+                const double da = qmean() - old;
+                if (absv(da) < p->eps) break;
+            }
+            if ((rc = dg_mstep(c, nullptr))) return rc;
+            if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
+        }
+        if (p->isDA && !isLast) {
+            if (p->schedule == DG_SCHED_LINEAR) temp -= p->delta;
+            else temp *= p->alpha;
+        } else {
+            break;
+        }
+    } while (!isLast);
+    res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
+    res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual; res->beta = c->beta;
+    return DG_OK;
+}
+
+int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_map before dg_upload");
+    if (!X || !M || L < 1 || L > 3 || ldX < L || ldM < L) return dg_fail(c, DG_EINVAL, "dg_map: bad args");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    if (!c->have_estep) {
+        dg_estep_out eo;
+        if ((rc = dg_estep(c, c->last_isDA, c->last_temp, &eo))) return rc;
+    }
+    double *dX = nullptr, *dM = nullptr;
+    if ((rc = dalloc(c, &dX, (size_t)c->K * L))) return rc;
+    if ((rc = dalloc(c, &dM, (size_t)c->N * L))) { dfree(dX); return rc; }
+    hipError_t e = hipMemcpy2DAsync(dX, (size_t)L * sizeof(double), X, (size_t)ldX * sizeof(double),
+                                    (size_t)L * sizeof(double), (size_t)c->K,
+                                    hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        // the mixed path keeps its FP64 state current, so the FP64 kernel serves both
+        rc = dg_map_fp64(c, dX, L, dM);
+        if (!rc)
+            e = hipMemcpy2DAsync(M, (size_t)ldM * sizeof(double), dM, (size_t)L * sizeof(double),
+                                 (size_t)L * sizeof(double), (size_t)c->N, hipMemcpyDeviceToHost,
+                                 c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    dfree(dX); dfree(dM);
+    if (e != hipSuccess) return dg_hip_check(c, e, "dg_map");
+    return rc;
+}
+
+int dg_resp_dense(dg_ctx *c, double *R, long ldR) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded || !c->have_estep) return dg_fail(c, DG_ESTATE, "dg_resp_dense needs a dg_estep");
+    if (!R || ldR < c->N) return dg_fail(c, DG_EINVAL, "dg_resp_dense: bad args");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    double *dR = nullptr;
+    if ((rc = dalloc(c, &dR, (size_t)c->K * c->N))) return rc;
+    rc = dg_resp_dense_fp64(c, dR, c->N);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpy2DAsync(R, (size_t)ldR * sizeof(double), dR, (size_t)c->N * sizeof(double),
+                             (size_t)c->N * sizeof(double), (size_t)c->K, hipMemcpyDeviceToHost,
+                             c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    dfree(dR);
+    if (e != hipSuccess) return dg_hip_check(c, e, "dg_resp_dense");
+    return rc;
+}
+
+int dg_download(dg_ctx *c, double *W, long ldW, double *Y, long ldY, double *beta) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_download before dg_upload");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    if (W) {
+        if (ldW < c->D) return dg_fail(c, DG_EINVAL, "dg_download: ldW < D");
+        DG_HIP(c, hipMemcpy2DAsync(W, (size_t)ldW * sizeof(double), c->W,
+                                   (size_t)c->D * sizeof(double), (size_t)c->D * sizeof(double),
+                                   (size_t)c->M1, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (Y) {
+        if (ldY < c->D) return dg_fail(c, DG_EINVAL, "dg_download: ldY < D");
+        DG_HIP(c, hipMemcpy2DAsync(Y, (size_t)ldY * sizeof(double), c->Y64,
+                                   (size_t)c->ldt * sizeof(double), (size_t)c->D * sizeof(double),
+                                   (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
+    }
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    if (beta) *beta = c->beta;
+    return DG_OK;
+}
+
+int dg_get_stats(dg_ctx *c, double *G, double *RT, long ldRT) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded || !c->have_estep) return dg_fail(c, DG_ESTATE, "dg_get_stats needs a dg_estep");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    if (G)
+        DG_HIP(c, hipMemcpyAsync(G, c->G, (size_t)c->K * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (RT) {
+        if (ldRT < c->D) return dg_fail(c, DG_EINVAL, "dg_get_stats: ldRT < D");
+        DG_HIP(c, hipMemcpy2DAsync(RT, (size_t)ldRT * sizeof(double), c->RT,
+                                   (size_t)c->D * sizeof(double), (size_t)c->D * sizeof(double),
+                                   (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
+    }
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+int dg_set_timing(dg_ctx *c, int on) {
+    if (!c) return DG_EINVAL;
+    c->timing = on ? 1 : 0;
+    return DG_OK;
+}
+
+int dg_get_timing(dg_ctx *c, double t[4]) {
+    if (!c || !t) return DG_EINVAL;
+    for (int i = 0; i < 4; ++i) t[i] = c->t_ms[i];
+    return DG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Internal declarations shared by the HIP translation units of libdagtm_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/dagtm_gpu.h"
+
+// exp(x) of an FP64 x below this value rounds to +0 (2^-1075 is the midpoint
+// under the smallest subnormal).  The reference applies exp before pow
+// (mpicomm.c:334, 350), so an element whose exponent -beta*d/2 falls below
+// it contributes nothing at any temperature; both precisions reproduce that.
+constexpr double kExpUnderflowX = -745.1332191019412;
+
+struct dg_ctx {
+    int device = 0;
+    dg_config cfg{};
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // problem shape
+    long N = 0;
+    int D = 0, K = 0, M1 = 0;
+    bool loaded = false;
+    bool have_estep = false;       // G/RT/pm/pl valid for the current Y
+    bool last_isDA = true;
+    double last_temp = 1.0;
+    double beta = 0.0;
+    double sum_tt = 0.0;           // sum_n |t_n|^2 (FP64)
+
+    // ---- FP64 state (always present) ----
+    int ldt = 0;                   // padded row stride of T64 / Y64
+    double *T64 = nullptr;         // N x ldt
+    double *tt = nullptr;          // N      |t_n|^2
+    double *FI = nullptr;          // K x M1
+    double *Y64 = nullptr;         // K x ldt
+    double *yy = nullptr;          // K      |y_k|^2
+    double *W = nullptr;           // M1 x D
+    double *A = nullptr;           // M1 x M1 (lower triangle holds the factor after a solve)
+    double *A0 = nullptr;          // M1 x M1 unfactored copy
+    double *Bm = nullptr;          // M1 x D
+    double *G = nullptr;           // K
+    double *RT = nullptr;          // K x D
+    double *pm = nullptr;          // N   per-point max exponent x_max (pass 1)
+    double *pl = nullptr;          // N   per-point log S2 (pass 1)
+    double *logg = nullptr;        // N
+    double *logh = nullptr;        // N
+    double *partRT = nullptr;      // NC x K x D
+    double *partG = nullptr;       // NC x K
+    int NC = 1;                    // N chunks of E-pass 2
+    double *scal = nullptr;        // device scalars (see kScal*)
+    double *h_scal = nullptr;      // pinned host mirror
+
+    // ---- mixed-precision state ----
+    int mixed_ready = 0;
+    int ldh = 0;                   // padded row stride of the 16-bit operands
+    int t_exact = 0;               // T exactly representable in the 16-bit type
+    float t_scale = 1.f;           // power-of-two scale applied to T and Y
+    void *mix = nullptr;           // padded shapes and split operands (dg_mixed.hip)
+    void *T16 = nullptr;           // TT x Np x ldh   FP16 terms of T
+    float *tt32 = nullptr;         // Np   |t'|^2
+    float *yy32 = nullptr;         // Kp   |y'|^2
+    float *pm32 = nullptr;         // Np   pass-2 exponent offset per point
+    float *pl32 = nullptr;         // Np   pass-2 underflow threshold per point
+    float *partRT32 = nullptr;     // NC x D x K
+    float *partG32 = nullptr;      // NC x K
+
+    // timing
+    int timing = 0;
+    hipEvent_t ev[6] = {};
+    double t_ms[4] = {0, 0, 0, 0};
+};
+
+// scalar slots in ctx->scal
+enum {
+    kScalSumLogG = 0,   // sum_n log g_n
+    kScalSumLogH = 1,   // sum_n log h_n
+    kScalNZero = 2,     // zero columns (as double)
+    kScalBetaSum = 3,   // sum R o dist(Y_new, T)
+    kScalJitter = 4,    // jitter used by the last factorisation
+    kScalSolveOK = 5,   // 1 if the factorisation succeeded
+    kScalCount = 8
+};
+
+int dg_fail(dg_ctx *ctx, int code, const std::string &msg);
+int dg_hip_check(dg_ctx *ctx, hipError_t e, const char *what);
+
+#define DG_HIP(ctx, call)                                             \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return dg_hip_check((ctx), e_, #call);  \
+    } while (0)
+
+// ---- FP64 kernels (dg_fp64.hip) ----
+int dg_estep_fp64(dg_ctx *ctx, int isDA, double temp);
+int dg_resp_dense_fp64(dg_ctx *ctx, double *dR, long ldR);
+int dg_map_fp64(dg_ctx *ctx, const double *dX, int L, double *dM);
+
+// ---- mixed kernels (dg_mixed.hip) ----
+int dg_mixed_prepare_t(dg_ctx *ctx);
+int dg_mixed_prepare_y(dg_ctx *ctx);
+int dg_estep_mixed(dg_ctx *ctx, int isDA, double temp);
+void dg_mixed_free(dg_ctx *ctx);
+
+// ---- M-step (dg_mstep.hip) ----
+int dg_mstep_run(dg_ctx *ctx);
+int dg_rownorms(dg_ctx *ctx, const double *X, long rows, int ld, int D, double *out);
+int dg_reduce_logs(dg_ctx *ctx);
+int dg_sum_into(dg_ctx *ctx, const double *v, long n, int slot);

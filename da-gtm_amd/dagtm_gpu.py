@@ -1,0 +1,195 @@
+"""ctypes binding of libdagtm_gpu.so (the C ABI declared in include/dagtm_gpu.h).
+
+The binding is thin on purpose: every call goes straight to the HIP library,
+and a missing or unloadable library raises instead of falling back to any
+CPU code.  Method names follow the reference's step names
+(dagtm/dagtm.cxx:1531-1795): ``estep`` = dist + resp + qual, ``mstep`` =
+rowsum + A/B + solve + FI W + beta, ``anneal`` = the temperature loop.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdagtm_gpu.so")
+
+DG_OK, DG_EINVAL, DG_ENOMEM, DG_EHIP, DG_ESTATE, DG_ESOLVE = 0, 1, 2, 3, 4, 5
+FP64, MIXED = 0, 1
+SCHED_EXP, SCHED_LINEAR = 0, 1
+
+EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
+           "dg_set_state", "dg_estep", "dg_mstep", "dg_anneal", "dg_map",
+           "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
+           "dg_get_timing"]
+
+
+class DgConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("lambda_", C.c_double),
+                ("y_terms", C.c_int), ("r_terms", C.c_int)]
+
+
+class DgEstepOut(C.Structure):
+    _fields_ = [("llh", C.c_double), ("nfree", C.c_double), ("nzero", C.c_long)]
+
+
+class DgAnnealParams(C.Structure):
+    _fields_ = [("isDA", C.c_int), ("schedule", C.c_int), ("maxloop", C.c_int),
+                ("alpha", C.c_double), ("delta", C.c_double), ("eps", C.c_double),
+                ("start_temp", C.c_double)]
+
+
+class DgAnnealResult(C.Structure):
+    _fields_ = [("nloop", C.c_int), ("ntemps", C.c_int), ("exit_maxloop", C.c_int),
+                ("temp", C.c_double), ("llh", C.c_double), ("nfree", C.c_double),
+                ("qual", C.c_double), ("beta", C.c_double)]
+
+
+ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                      C.c_double, C.c_double, C.c_double)
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+
+class DgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dagtm_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libdagtm_gpu.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libdagtm_gpu.so not built: run `make -C da-gtm_amd`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.dg_create.argtypes = [C.POINTER(DgConfig), C.POINTER(vp)]
+    L.dg_destroy.argtypes = [vp]
+    L.dg_last_error.argtypes = [vp]
+    L.dg_last_error.restype = C.c_char_p
+    L.dg_version.restype = C.c_char_p
+    L.dg_upload.argtypes = [vp, _dp, C.c_long, C.c_int, C.c_long, _dp, C.c_int, C.c_int,
+                            C.c_long, _dp, C.c_long, C.c_double]
+    L.dg_set_state.argtypes = [vp, _dp, C.c_long, C.c_double]
+    L.dg_estep.argtypes = [vp, C.c_int, C.c_double, C.POINTER(DgEstepOut)]
+    L.dg_mstep.argtypes = [vp, C.POINTER(C.c_double)]
+    L.dg_anneal.argtypes = [vp, C.POINTER(DgAnnealParams), C.POINTER(DgAnnealResult),
+                            ITER_CB, vp]
+    L.dg_map.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
+    L.dg_resp_dense.argtypes = [vp, _dp, C.c_long]
+    L.dg_download.argtypes = [vp, vp, C.c_long, vp, C.c_long, C.POINTER(C.c_double)]
+    L.dg_get_stats.argtypes = [vp, vp, vp, C.c_long]
+    L.dg_set_timing.argtypes = [vp, C.c_int]
+    L.dg_get_timing.argtypes = [vp, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Engine:
+    """One DA-GTM problem resident on one GPU."""
+
+    def __init__(self, precision=FP64, lam=0.0, device=0, y_terms=2, r_terms=2):
+        self.L = load()
+        cfg = DgConfig(device=device, precision=precision, lambda_=lam,
+                       y_terms=y_terms, r_terms=r_terms)
+        h = C.c_void_p()
+        rc = self.L.dg_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise DgError(rc, "dg_create failed (no usable HIP device?)")
+        self.h = h
+        self.precision = precision
+        self.N = self.D = self.K = self.M1 = 0
+
+    def _chk(self, rc):
+        if rc:
+            raise DgError(rc, self.L.dg_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, T, FI, Y, beta):
+        T, FI, Y = _f64(T), _f64(FI), _f64(Y)
+        N, D = T.shape
+        K, M1 = FI.shape
+        self._chk(self.L.dg_upload(self.h, T, N, D, D, FI, K, M1, M1, Y, D, float(beta)))
+        self.N, self.D, self.K, self.M1 = N, D, K, M1
+
+    def set_state(self, Y, beta):
+        Y = _f64(Y)
+        self._chk(self.L.dg_set_state(self.h, Y, Y.shape[1], float(beta)))
+
+    def estep(self, isDA=True, temp=1.0):
+        o = DgEstepOut()
+        self._chk(self.L.dg_estep(self.h, int(isDA), float(temp), C.byref(o)))
+        return o.llh, o.nfree, o.nzero
+
+    def mstep(self):
+        b = C.c_double()
+        self._chk(self.L.dg_mstep(self.h, C.byref(b)))
+        return b.value
+
+    def anneal(self, isDA=True, schedule=SCHED_EXP, maxloop=0, alpha=0.99, delta=0.1,
+               eps=1e-6, start_temp=1.0):
+        p = DgAnnealParams(isDA=int(isDA), schedule=schedule, maxloop=maxloop, alpha=alpha,
+                           delta=delta, eps=eps, start_temp=start_temp)
+        r = DgAnnealResult()
+        rows = []
+
+        def _cb(_u, nloop, lloop, temp, nfree, llh, diff, beta):
+            rows.append((nloop, lloop, temp, nfree, llh, diff, beta))
+
+        cb = ITER_CB(_cb)
+        self._chk(self.L.dg_anneal(self.h, C.byref(p), C.byref(r), cb, None))
+        return r, rows
+
+    def map(self, X):
+        X = _f64(X)
+        Mo = np.zeros((self.N, X.shape[1]))
+        self._chk(self.L.dg_map(self.h, X, X.shape[1], X.shape[1], Mo, X.shape[1]))
+        return Mo
+
+    def resp_dense(self):
+        R = np.zeros((self.K, self.N))
+        self._chk(self.L.dg_resp_dense(self.h, R, self.N))
+        return R
+
+    def download(self):
+        W = np.zeros((self.M1, self.D)); Y = np.zeros((self.K, self.D)); b = C.c_double()
+        self._chk(self.L.dg_download(self.h, W.ctypes.data, self.D, Y.ctypes.data, self.D,
+                                     C.byref(b)))
+        return W, Y, b.value
+
+    def stats(self):
+        G = np.zeros(self.K); RT = np.zeros((self.K, self.D))
+        self._chk(self.L.dg_get_stats(self.h, G.ctypes.data, RT.ctypes.data, self.D))
+        return G, RT
+
+    def set_timing(self, on=True):
+        self._chk(self.L.dg_set_timing(self.h, int(on)))
+
+    def timing(self):
+        t = (C.c_double * 4)()
+        self._chk(self.L.dg_get_timing(self.h, t))
+        return list(t)

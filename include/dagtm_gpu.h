@@ -1,0 +1,144 @@
+/*
+ * dagtm_gpu.h -- C ABI of the MI355X DA-GTM engine (libdagtm_gpu.so).
+ *
+ * The reference program (jychoi-hpc/da-gtm) has no plugin or FFI layer: its
+ * hot path is a set of internal C functions of libdagtmcomm called from
+ * dagtm/dagtm.cxx with GSL matrices and MPI communicators.  Each entry point
+ * below replaces one of those call sites on a 1x1 process grid (one GPU holds
+ * the whole K x N problem, so the communicators disappear).  The replaced
+ * reference interface is cited per function.
+ *
+ * Conventions (mirroring the reference's, comm.h:122-131):
+ *   - every function returns 0 (DG_OK) or a positive DG_E* code and never
+ *     aborts; dg_last_error() gives the message of the last failure;
+ *   - matrices are row-major doubles with an explicit row stride (ld*), so a
+ *     padded gsl_matrix (tda, comm.c:1866-1883) can be passed directly;
+ *   - host buffers are borrowed only for the duration of a call; device
+ *     buffers belong to the context; calls are synchronous at the boundary;
+ *   - one context per host thread; a context owns one HIP stream.
+ */
+#ifndef DAGTM_GPU_H
+#define DAGTM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG_OK       0
+#define DG_EINVAL   1   /* bad argument or unsupported shape */
+#define DG_ENOMEM   2   /* device allocation failed */
+#define DG_EHIP     3   /* HIP runtime error (no device, launch failure) */
+#define DG_ESTATE   4   /* call out of order (e.g. mstep before estep) */
+#define DG_ESOLVE   5   /* M-step factorisation failed even with jitter */
+
+typedef struct dg_ctx dg_ctx;
+
+enum dg_precision {
+    DG_FP64  = 0,   /* every step in FP64 (the FP64 GPU oracle) */
+    DG_MIXED = 1    /* 16-bit split MFMA distances, FP32 epilogue, FP64 M-step */
+};
+
+typedef struct {
+    int    device;      /* HIP device ordinal */
+    int    precision;   /* enum dg_precision */
+    double lambda;      /* M-step ridge, A + (lambda/beta) I; 0 = reference */
+    int    y_terms;     /* DG_MIXED: 16-bit terms per centre coordinate (2|3) */
+    int    r_terms;     /* DG_MIXED: 16-bit terms per responsibility (1|2) */
+} dg_config;
+
+/* Replaces the set-up of dagtm.cxx:893-1068 (MPI_Init, grid, allocation). */
+int  dg_create(const dg_config *cfg, dg_ctx **out);
+int  dg_destroy(dg_ctx *ctx);
+const char *dg_last_error(const dg_ctx *ctx);
+/* Version / build string of the library. */
+const char *dg_version(void);
+
+/* Copies the problem to the device.  Replaces h5read_byrow of T
+ * (dagtm.cxx:1076-1102) and the initial dagtm_dist (dagtm.cxx:1232):
+ *   T  N x D   (ldT >= D)      data
+ *   FI K x M1  (ldFI >= M1)    basis matrix, M1 = M + 1
+ *   Y  K x D   (ldY >= D)      centres (FI W0, or Y from a checkpoint)
+ *   beta                       inverse noise variance */
+int dg_upload(dg_ctx *ctx, const double *T, long N, int D, long ldT,
+              const double *FI, int K, int M1, long ldFI,
+              const double *Y, long ldY, double beta);
+
+/* Replaces the restart read of Y and beta (dagtm.cxx:1161-1213). */
+int dg_set_state(dg_ctx *ctx, const double *Y, long ldY, double beta);
+
+typedef struct {
+    double llh;        /* normalised log-likelihood (mpicomm.c:463-500, from g) */
+    double nfree;      /* normalised -F (from h; equals -llh in EM mode)        */
+    long   nzero;      /* columns that took the zero-column fall-back (Q2)      */
+} dg_estep_out;
+
+/* One E-step at the context's beta: distances, responsibilities and both
+ * qualities.  Replaces dagtm_dist_ws (comm.c:560-629) for the current Y,
+ * dagtm_mpi_resp (mpicomm.c:316-414) and the two dagtm_mpi_qual calls
+ * (mpicomm.c:463-500) of dagtm.cxx:1537-1603.  R is never materialised:
+ * the call leaves G = rowsum(R) and RT = R T on the device for dg_mstep. */
+int dg_estep(dg_ctx *ctx, int isDA, double temp, dg_estep_out *out);
+
+/* M-step and beta update: G, A = FI^T diag(G) FI (+ lambda/beta I),
+ * B = FI^T (R T), W by Cholesky, Y = FI W and beta = N D / sum R o dist.
+ * Replaces dagtm.cxx:1715-1785 (rowsum, dgemm, dagtm_solve2, dgemm) and
+ * dagtm_mpi_beta (mpicomm.c:276-302).  *beta_out may be NULL. */
+int dg_mstep(dg_ctx *ctx, double *beta_out);
+
+enum { DG_SCHED_EXP = 0, DG_SCHED_LINEAR = 1 };
+
+typedef struct {
+    int    isDA;        /* 1 = deterministic annealing, 0 = plain EM (-m) */
+    int    schedule;    /* DG_SCHED_EXP (-c exp) or DG_SCHED_LINEAR (-c linear) */
+    int    maxloop;     /* -j, 0 = unlimited */
+    double alpha;       /* -a, EXP cooling factor */
+    double delta;       /* LINEAR step (dagtm.cxx:2169-2171) */
+    double eps;         /* -e, convergence threshold on |diff| */
+    double start_temp;  /* -S or the first critical temperature / alpha^10 */
+} dg_anneal_params;
+
+typedef struct {
+    int    nloop;        /* total E+M iterations */
+    int    ntemps;       /* temperatures visited */
+    int    exit_maxloop; /* 1 when the run ended on maxloop */
+    double temp, llh, nfree, qual, beta;
+} dg_anneal_result;
+
+/* Progress row of dagtm.cxx:1570-1571, called after every E-step. */
+typedef void (*dg_iter_cb)(void *user, int nloop, int lloop, double temp,
+                           double nfree, double llh, double diff,
+                           double beta);
+
+/* The annealing loop dagtm.cxx:1508-2206 for the EXP and LINEAR schedules
+ * and EM mode, with the reference's convergence tests (|diff| < eps, the
+ * 50-iteration moving average, qual_old = 0 at start). */
+int dg_anneal(dg_ctx *ctx, const dg_anneal_params *p, dg_anneal_result *res,
+              dg_iter_cb cb, void *user);
+
+/* MAP projection M = R^T X (N x L, ldM >= L) with the responsibilities of
+ * the last E-step.  Replaces dagtm.cxx:2347-2355. */
+int dg_map(dg_ctx *ctx, const double *X, int L, long ldX, double *M,
+           long ldM);
+
+/* Materialises R (K x N, ldR >= N) of the last E-step.  Debug / -R output
+ * (dagtm.cxx:2400-2414); intended for small problems. */
+int dg_resp_dense(dg_ctx *ctx, double *R, long ldR);
+
+/* Copies the model back.  Replaces the h5save_byrow of W, Y and beta
+ * (dagtm.cxx:2420-2480).  Any pointer may be NULL. */
+int dg_download(dg_ctx *ctx, double *W, long ldW, double *Y, long ldY,
+                double *beta);
+
+/* Copies G (K) and RT (K x D, ldRT >= D) of the last E-step (testing). */
+int dg_get_stats(dg_ctx *ctx, double *G, double *RT, long ldRT);
+
+/* Device time of the last dg_estep / dg_mstep, per kernel phase, in ms:
+ * t[0] E-pass 1, t[1] E-pass 2, t[2] reductions, t[3] M-step.  Timing is
+ * recorded only after dg_set_timing(ctx, 1). */
+int dg_set_timing(dg_ctx *ctx, int on);
+int dg_get_timing(dg_ctx *ctx, double t[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
