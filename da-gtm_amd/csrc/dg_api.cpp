@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -51,7 +52,7 @@ void free_problem(dg_ctx *c) {
     dfree(c->T64); dfree(c->tt); dfree(c->FI); dfree(c->Y64); dfree(c->yy);
     dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm); dfree(c->G); dfree(c->RT);
     dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
-    dfree(c->partRT); dfree(c->partG);
+    dfree(c->partRT); dfree(c->partG); dfree(c->partAB);
     dg_mixed_free(c);
     c->loaded = false;
     c->have_estep = false;
@@ -90,15 +91,13 @@ const char *dg_last_error(const dg_ctx *ctx) {
 int dg_create(const dg_config *cfg, dg_ctx **out) {
     if (!out) return DG_EINVAL;
     *out = nullptr;
-    dg_config def{0, DG_FP64, 0.0, 2, 2};
+    dg_config def{0, DG_FP64, 0.0};
     const dg_config &cf = cfg ? *cfg : def;
     if (cf.precision != DG_FP64 && cf.precision != DG_MIXED) return DG_EINVAL;
     if (cf.lambda < 0.0) return DG_EINVAL;
     dg_ctx *c = new (std::nothrow) dg_ctx();
     if (!c) return DG_ENOMEM;
     c->cfg = cf;
-    if (c->cfg.y_terms < 2 || c->cfg.y_terms > 3) c->cfg.y_terms = 2;
-    if (c->cfg.r_terms < 1 || c->cfg.r_terms > 2) c->cfg.r_terms = 2;
     c->device = cf.device;
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -162,6 +161,10 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     if (nc > tiles) nc = tiles;
     if (nc < 1) nc = 1;
     c->NC = (int)nc;
+    // A/B reduction: at most 64 chunks of centres, at least 64 centres each
+    long nch = (K + 63) / 64;
+    if (nch > 64) nch = 64;
+    c->nchAB = (int)(nch < 1 ? 1 : nch);
 
     if ((rc = dalloc(c, &c->T64, (size_t)N * ld)) || (rc = dalloc(c, &c->tt, N)) ||
         (rc = dalloc(c, &c->FI, (size_t)K * M1)) || (rc = dalloc(c, &c->Y64, (size_t)K * ld)) ||
@@ -171,7 +174,9 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
         (rc = dalloc(c, &c->RT, (size_t)K * D)) || (rc = dalloc(c, &c->pm, N)) ||
         (rc = dalloc(c, &c->pl, N)) || (rc = dalloc(c, &c->logg, N)) ||
         (rc = dalloc(c, &c->logh, N)) || (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
-        (rc = dalloc(c, &c->partG, (size_t)c->NC * K))) {
+        (rc = dalloc(c, &c->partG, (size_t)c->NC * K)) ||
+        (rc = dalloc(c, &c->partAB, std::max((size_t)c->nchAB * ((size_t)M1 * M1 + (size_t)M1 * D),
+                                             (size_t)(K + 255) / 256)))) {
         free_problem(c);
         return rc;
     }
@@ -330,6 +335,10 @@ int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
         dg_estep_out eo;
         if ((rc = dg_estep(c, c->last_isDA, c->last_temp, &eo))) return rc;
     }
+    // output projections run in FP64; the mixed path refreshes the FP64
+    // per-point normalisers from the same model state first
+    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+        return rc;
     double *dX = nullptr, *dM = nullptr;
     if ((rc = dalloc(c, &dX, (size_t)c->K * L))) return rc;
     if ((rc = dalloc(c, &dM, (size_t)c->N * L))) { dfree(dX); return rc; }
@@ -356,6 +365,8 @@ int dg_resp_dense(dg_ctx *c, double *R, long ldR) {
     if (!R || ldR < c->N) return dg_fail(c, DG_EINVAL, "dg_resp_dense: bad args");
     int rc;
     if ((rc = select_device(c))) return rc;
+    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+        return rc;
     double *dR = nullptr;
     if ((rc = dalloc(c, &dR, (size_t)c->K * c->N))) return rc;
     rc = dg_resp_dense_fp64(c, dR, c->N);

@@ -51,6 +51,8 @@ struct dg_ctx {
     double *partRT = nullptr;      // NC x K x D
     double *partG = nullptr;       // NC x K
     int NC = 1;                    // N chunks of E-pass 2
+    double *partAB = nullptr;      // nchAB x (M1*M1 + M1*D) split-K partials of A, B
+    int nchAB = 1;                 // centre chunks of the A/B reduction
     double *scal = nullptr;        // device scalars (see kScal*)
     double *h_scal = nullptr;      // pinned host mirror
 

@@ -23,6 +23,8 @@
 // R goes from the exponential straight into the second MFMA as an FP16
 // hi/lo pair without touching LDS.  Each chunk writes FP32 partials that a
 // deterministic FP64 pass sums.
+#include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "dg_internal.h"
@@ -43,6 +45,7 @@ constexpr int PAD = 8;            // LDS row padding (halves)
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kZmin = -1075.0f;  // log2 of the FP64 exp underflow point
 constexpr float kRLog2 = 14.0f;    // R is carried as R * 2^14 in FP16
+constexpr float kMaskMargin = 64.0f;  // log2 distance beyond which masking is moot
 constexpr double kLn2 = 0.6931471805599453;
 
 __device__ __forceinline__ f16v mfma(h8 a, h8 b, f16v c) {
@@ -65,7 +68,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp,
     const double *__restrict__ mscal, float beta, float it, int two, int isDA, double logK,
     float *__restrict__ pB, float *__restrict__ pThr, double *__restrict__ pm,
-    double *__restrict__ logg, double *__restrict__ logh) {
+    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;
     constexpr int ROW8 = ld / 8;                 // h8 chunks per row
@@ -73,12 +76,17 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     constexpr int PER = (CH + NTH - 1) / NTH;
     __shared__ __attribute__((aligned(16))) _Float16 Ys[2][2][KT][LR];
     __shared__ float aKs[2][KT];
+    __shared__ double wsum[3][4];
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
     const long n = (long)blockIdx.x * PT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
+    // exponents are carried without this point's constant bn = cb |t_n|^2:
+    // z' = z - bn; exp underflow (z < kZmin) is z' < thr
     const float bn = cb * tt32[n];
+    const float thr = kZmin - bn;
+    const float mmargin = kMaskMargin / it;
 
     h8 tb[TT][KS];
 #pragma unroll
@@ -114,7 +122,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         if (tid < KT) aKs[buf][tid] = apre;
     };
 
-    float m = kZmin, S1 = 0.f, C1 = 0.f, S2 = 0.f, C2 = 0.f;
+    float m = thr, S1 = 0.f, C1 = 0.f, S2 = 0.f, C2 = 0.f;
     const int ntiles = Kp / KT;
     gload(0);
     lstore(0);
@@ -138,12 +146,22 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-            float zz = fmaf(c1, acc[r], aKs[buf][i] + bn);
-            zz = (zz < kZmin) ? -INFINITY : zz;
-            z[r] = zz;
-            zt = fmaxf(zt, zz);
+            z[r] = fmaf(c1, acc[r], aKs[buf][i]);
+            zt = fmaxf(zt, z[r]);
         }
         zt = fmaxf(zt, __shfl_xor(zt, 32));
+        // The underflow mask only matters when a masked element could weigh
+        // more than 2^-64 of the running sum, i.e. when the tile maximum is
+        // within kMaskMargin * temp of the threshold; otherwise skip it.
+        if (!__all(zt - thr > mmargin)) {
+            zt = -INFINITY;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                z[r] = (z[r] < thr) ? -INFINITY : z[r];
+                zt = fmaxf(zt, z[r]);
+            }
+            zt = fmaxf(zt, __shfl_xor(zt, 32));
+        }
         if (zt > m) {
             const float dm = m - zt;
             const float s1 = ex2(dm);
@@ -155,11 +173,11 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             m = zt;
         }
         float t1 = 0.f, t2 = 0.f;
+        const float nmit = -m * it;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const float dz = z[r] - m;
-            t1 += ex2(dz);
-            if (two) t2 += ex2(dz * it);
+            t1 += ex2(z[r] - m);
+            if (two) t2 += ex2(fmaf(z[r], it, nmit));
         }
         kahan(S1, C1, t1);
         if (two) kahan(S2, C2, t2);
@@ -169,6 +187,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     // lanes j and j+32 hold the same point (different centre rows) and share m
     const float S1o = __shfl_xor(S1, 32), C1o = __shfl_xor(C1, 32);
     const float S2o = __shfl_xor(S2, 32), C2o = __shfl_xor(C2, 32);
+    double lg = 0.0, lh = 0.0, zc = 0.0;      // this lane's share of the workgroup sums
     if (h == 0) {
         const double s1 = ((double)S1 - C1) + ((double)S1o - C1o);
         const double s2 = two ? ((double)S2 - C2) + ((double)S2o - C2o) : s1;
@@ -176,70 +195,119 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             pB[n] = 0.f;
             pThr[n] = INFINITY;
         } else if (!(s1 > 0.0)) {          // zero column (mpicomm.c:382-390)
+            lg = isDA ? -INFINITY : logK;
+            lh = logK;
+            zc = 1.0;
             pm[n] = -INFINITY;
-            logg[n] = isDA ? -INFINITY : logK;
-            logh[n] = logK;
+            logg[n] = lg;
+            logh[n] = lh;
             pB[n] = 0.f;
             pThr[n] = INFINITY;            // pass 2 skips it; a fix-up adds 1/K
         } else {
             const double L1 = log2(s1), L2 = log2(s2);
-            pm[n] = (double)m;
-            logg[n] = kLn2 * ((double)m + L1);
-            logh[n] = kLn2 * ((double)m * it + L2);
-            pB[n] = (float)((double)it * ((double)bn - m) - L2 + kRLog2);
-            pThr[n] = (float)((double)it * ((double)kZmin - m) - L2 + kRLog2);
+            const double ma = (double)m + (double)bn;        // the point's max exponent
+            lg = kLn2 * (ma + L1);
+            lh = kLn2 * (ma * it + L2);
+            pm[n] = ma;
+            logg[n] = lg;
+            logh[n] = lh;
+            pB[n] = (float)(-(double)it * m - L2 + kRLog2);
+            pThr[n] = (float)((double)it * ((double)thr - m) - L2 + kRLog2);
         }
+    }
+    // workgroup partials of sum log g, sum log h and the zero-column count
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        lg += __shfl_xor(lg, o);
+        lh += __shfl_xor(lh, o);
+        zc += __shfl_xor(zc, o);
+    }
+    if (l == 0) { wsum[0][w] = lg; wsum[1][w] = lh; wsum[2][w] = zc; }
+    __syncthreads();
+    if (tid < 3)
+        partLog[(long)blockIdx.x * 3 + tid] =
+            (wsum[tid][0] + wsum[tid][1]) + (wsum[tid][2] + wsum[tid][3]);
+}
+
+// scal[0..2] = sums over the pass-1 workgroup partials (fixed tree order).
+__global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__ part, long nb,
+                                                      double *__restrict__ scal) {
+    __shared__ double s[3][1024];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long i = threadIdx.x; i < nb; i += 1024) {
+        a += part[3 * i]; b += part[3 * i + 1]; c += part[3 * i + 2];
+    }
+    s[0][threadIdx.x] = a; s[1][threadIdx.x] = b; s[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+            s[2][threadIdx.x] += s[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[kScalSumLogG] = s[0][0];
+        scal[kScalSumLogH] = s[1][0];
+        scal[kScalNZero] = s[2][0];
     }
 }
 
 // ---------------------------------------------------------------- pass 2
+// Per 32-point block: LDS-staged data (double-buffered, the next block
+// prefetched into registers), cross-term MFMAs, the exponential epilogue and
+// the R T MFMAs run in one straight-line region, so every read of an MFMA
+// result stays in the basic block of the MFMA that produced it.  The 1-D
+// grid is mapped so that the workgroups the hardware places on one XCD
+// (linear id mod 8) share a point chunk and walk all its centre groups,
+// keeping that chunk's data in the XCD's L2.
 template <int KS, int TT>
 __global__ __launch_bounds__(NTH) void k_e2_mfma(
     const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it,
-    const float *__restrict__ pB, const float *__restrict__ pThr, long chunk,
+    const float *__restrict__ pB, const float *__restrict__ pThr, long chunk, int KG,
     float *__restrict__ partRT, float *__restrict__ partG) {
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;
-    constexpr int DB = (ld + 31) / 32;          // 32-row blocks of d
-    constexpr int DR = 32 * DB;                 // rows of the transposed data
-    constexpr int TR = BP + PAD;                // transposed LDS row (halves)
+    constexpr int DB = (ld + 31) / 32;
+    constexpr int DR = 32 * DB;
+    constexpr int TR = BP + PAD;
     constexpr int ROW8 = ld / 8;
-    constexpr int CHA = TT * BP * ROW8;         // h8 chunks: row-major block
-    constexpr int CHB = TT * DR * (BP / 8);     // h8 chunks: transposed block
+    constexpr int CHA = TT * BP * ROW8;
+    constexpr int CHB = TT * DR * (BP / 8);
     constexpr int PA = (CHA + NTH - 1) / NTH;
     constexpr int PB_ = (CHB + NTH - 1) / NTH;
     __shared__ __attribute__((aligned(16))) _Float16 Ts[2][TT][BP][LR];
     __shared__ __attribute__((aligned(16))) _Float16 Tts[2][TT][DR][TR];
     __shared__ __attribute__((aligned(16))) float pBs[2][BP];
     __shared__ __attribute__((aligned(16))) float pTs[2][BP];
-
+    const int L = blockIdx.x;
+    const int xs = L & 7, sl = L >> 3;
+    const int kg = sl % KG;
+    const long cidx = (long)(sl / KG) * 8 + xs;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
-    const int k = blockIdx.x * CT + 32 * w + j;
+    const int k = kg * CT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
     const float itc1 = it * c1;
     const float aKi = (k < K) ? it * (cb * yy32[k]) : -INFINITY;
-
     h8 yb[2][KS];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
             yb[t][ks] = *(const h8 *)(Y16 + ((long)t * Kp + k) * ld + 16 * ks + 8 * h);
-
     f16v acc[DB];
 #pragma unroll
     for (int b = 0; b < DB; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
     float G = 0.f, Gc = 0.f;
-
-    const long nb = (long)blockIdx.y * chunk;
+    const long nb = cidx * chunk;
     const long ne = (nb + chunk < Np) ? nb + chunk : Np;
     const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
-
     h8 preA[PA], preB[PB_];
     float pp = 0.f;
     auto gload = [&](long n0) {
@@ -286,16 +354,11 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
         if (tid < BP) pBs[buf][tid] = pp;
         else if (tid < 2 * BP) pTs[buf][tid - BP] = pp;
     };
-
-    if (nblocks > 0) {
-        gload(nb);
-        lstore(0);
-    }
+    if (nblocks > 0) { gload(nb); lstore(0); }
     __syncthreads();
     for (int b = 0; b < nblocks; ++b) {
         const int buf = b & 1;
         if (b + 1 < nblocks) gload(nb + (long)(b + 1) * BP);
-        // cross term, rows = points, columns = this wave's centres
         f16v c;
 #pragma unroll
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
@@ -309,29 +372,41 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
                 c = mfma(a1, yb[0][ks], c);
             }
         }
-        // responsibilities (scaled by 2^14) into the B operand of the next MFMA
-        h8 rhi[2], rlo[2];
-        float gs = 0.f;
+        // Exponents first, in the basic block of the MFMAs that produced c.
+        // Elements below a point's threshold carry less than 2^-64 when the
+        // threshold itself is below -64, so the underflow mask (a VALU-only
+        // block) runs only for blocks that need it.
+        float e[16];
+        float ptmax = -INFINITY;
 #pragma unroll
         for (int m4 = 0; m4 < 4; ++m4) {
             const f4v pb = *(const f4v *)&pBs[buf][8 * m4 + 4 * h];
             const f4v pt = *(const f4v *)&pTs[buf][8 * m4 + 4 * h];
+            ptmax = fmaxf(ptmax, fmaxf(fmaxf(pt[0], pt[1]), fmaxf(pt[2], pt[3])));
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = 4 * m4 + q;
-                float e = fmaf(itc1, c[r], aKi) + pb[q];
-                e = (e < pt[q]) ? -INFINITY : e;
-                const float R = ex2(e);
-                gs += R;
-                const _Float16 hi = (_Float16)R;
-                const _Float16 lo = (_Float16)(R - (float)hi);
-                rhi[r >> 3][r & 7] = hi;
-                rlo[r >> 3][r & 7] = lo;
+            for (int q = 0; q < 4; ++q) e[4 * m4 + q] = fmaf(itc1, c[4 * m4 + q], aKi) + pb[q];
+        }
+        if (!__all(ptmax < -kMaskMargin)) {
+#pragma unroll
+            for (int m4 = 0; m4 < 4; ++m4) {
+                const f4v pt = *(const f4v *)&pTs[buf][8 * m4 + 4 * h];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    e[4 * m4 + q] = (e[4 * m4 + q] < pt[q]) ? -INFINITY : e[4 * m4 + q];
             }
         }
+        h8 rhi[2], rlo[2];
+        float gs = 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float R = ex2(e[r]);
+            gs += R;
+            const _Float16 hi = (_Float16)R;
+            const _Float16 lo = (_Float16)(R - (float)hi);
+            rhi[r >> 3][r & 7] = hi;
+            rlo[r >> 3][r & 7] = lo;
+        }
         kahan(G, Gc, gs);
-        // RT^T[d][k] += sum_n T^T[d][n] R[n][k]; inner index of MFMA s, lane half h,
-        // slot q maps to point 16 s + 8 (q / 4) + 4 h + q % 4 on both operands.
 #pragma unroll
         for (int db = 0; db < DB; ++db) {
 #pragma unroll
@@ -352,9 +427,8 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
         if (b + 1 < nblocks) lstore(buf ^ 1);
         __syncthreads();
     }
-
-    const float inv = 1.0f / 16384.0f;
-    const long cidx = blockIdx.y;
+    const float inv = (float)(1.0 / (16384.0 * mscal[1]));
+    const float ginv = 1.0f / 16384.0f;
     if (k < K) {
 #pragma unroll
         for (int db = 0; db < DB; ++db)
@@ -365,7 +439,7 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
             }
     }
     const float Go = __shfl_xor(G, 32), Gco = __shfl_xor(Gc, 32);
-    if (h == 0 && k < K) partG[cidx * K + k] = (((G - Gc) + (Go - Gco))) * inv;
+    if (h == 0 && k < K) partG[cidx * K + k] = (((G - Gc) + (Go - Gco))) * ginv;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -414,14 +488,15 @@ __global__ __launch_bounds__(256) void k_zero_fix(const double *__restrict__ sca
     }
 }
 
-// max |x| and the count of values not exact in FP16 over rows x D (stride ld),
-// single workgroup, into out[0], out[1].
-__global__ __launch_bounds__(1024) void k_absmax_exact(const double *__restrict__ X, long rows,
-                                                       int ld, int D, double *__restrict__ out) {
-    __shared__ double smax[1024], scnt[1024];
+// Workgroup partials of max |x| and of the count of values not exact in
+// FP16 over rows x D (stride ld): part[2 b], part[2 b + 1].
+constexpr int kAmaxWG = 256;
+__global__ __launch_bounds__(256) void k_absmax_part(const double *__restrict__ X, long rows,
+                                                     int ld, int D, double *__restrict__ part) {
+    __shared__ double smax[256], scnt[256];
     double mx = 0.0, cnt = 0.0;
     const long total = rows * D;
-    for (long e = threadIdx.x; e < total; e += 1024) {
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
         const double v = X[(e / D) * ld + (e % D)];
         mx = fmax(mx, fabs(v));
         const _Float16 hv = (_Float16)(float)v;
@@ -429,7 +504,24 @@ __global__ __launch_bounds__(1024) void k_absmax_exact(const double *__restrict_
     }
     smax[threadIdx.x] = mx; scnt[threadIdx.x] = cnt;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            scnt[threadIdx.x] += scnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = smax[0]; part[2 * blockIdx.x + 1] = scnt[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_absmax_final(const double *__restrict__ part, int n,
+                                                      double *__restrict__ out) {
+    __shared__ double smax[256], scnt[256];
+    double mx = 0.0, cnt = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) { mx = fmax(mx, part[2 * i]); cnt += part[2 * i + 1]; }
+    smax[threadIdx.x] = mx; scnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
             scnt[threadIdx.x] += scnt[threadIdx.x + w];
@@ -525,6 +617,8 @@ struct MixedBufs {
     _Float16 *T16t = nullptr;
     _Float16 *Y16 = nullptr;
     double *mscal = nullptr;
+    double *partLog = nullptr;    // 3 per pass-1 workgroup
+    double *amaxPart = nullptr;   // 2 x kAmaxWG
     long Np = 0;
     int Kp = 0, KS = 0, TT = 1, NC = 1;
 };
@@ -537,14 +631,15 @@ void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int is
     hipLaunchKernelGGL((k_e1_mfma<KS, TT>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0, c->stream,
                        (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16, c->yy32, c->K,
                        b->Kp, b->mscal, beta, it, two, isDA, logK, c->pm32, c->pl32, c->pm,
-                       c->logg, c->logh);
+                       c->logg, c->logh, b->partLog);
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
     const long blocks = b->Np / BP;
     const long chunk = ((blocks + b->NC - 1) / b->NC) * BP;
-    hipLaunchKernelGGL((k_e2_mfma<KS, TT>), dim3((unsigned)(b->Kp / CT), (unsigned)b->NC),
-                       dim3(NTH), 0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
-                       c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pm32, c->pl32, chunk,
-                       c->partRT32, c->partG32);
+    const int KG = b->Kp / CT;
+    hipLaunchKernelGGL((k_e2_mfma<KS, TT>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0, c->stream,
+                       (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32, c->K, b->Kp,
+                       c->D, b->mscal, beta, it, c->pm32, c->pl32, chunk, KG, c->partRT32,
+                       c->partG32);
 }
 
 template <int TT>
@@ -556,8 +651,17 @@ int dispatch_ks(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int isDA
         case 4: launch_mixed<4, TT>(c, b, beta, it, two, isDA, logK); break;
         case 6: launch_mixed<6, TT>(c, b, beta, it, two, isDA, logK); break;
         case 8: launch_mixed<8, TT>(c, b, beta, it, two, isDA, logK); break;
-        case 11: launch_mixed<11, TT>(c, b, beta, it, two, isDA, logK); break;
-        case 12: launch_mixed<12, TT>(c, b, beta, it, two, isDA, logK); break;
+        case 11:
+        case 12:
+            // the split data's two pass-2 LDS rings exceed 160 KB above D = 128
+            if constexpr (TT == 1) {
+                if (b->KS == 11) launch_mixed<11, 1>(c, b, beta, it, two, isDA, logK);
+                else launch_mixed<12, 1>(c, b, beta, it, two, isDA, logK);
+                break;
+            } else {
+                return dg_fail(c, DG_EINVAL,
+                               "mixed path: data not exact in FP16 is limited to D <= 128");
+            }
         default: return dg_fail(c, DG_EINVAL, "mixed path: unsupported D");
     }
     return DG_OK;
@@ -583,6 +687,8 @@ void dg_mixed_free(dg_ctx *c) {
         if (b->T16t) (void)hipFree(b->T16t);
         if (b->Y16) (void)hipFree(b->Y16);
         if (b->mscal) (void)hipFree(b->mscal);
+        if (b->partLog) (void)hipFree(b->partLog);
+        if (b->amaxPart) (void)hipFree(b->amaxPart);
         delete b;
     }
     c->mix = nullptr;
@@ -618,12 +724,17 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     if (nc < want) nc = want;
     if (nc > blocks) nc = blocks;
     if (nc < 1) nc = 1;
+    nc = ((nc + 7) / 8) * 8;          // the pass-2 grid maps 8 chunks per XCD round
     b->NC = (int)nc;
 
     int rc;
+    if ((rc = dalloc(c, &b->amaxPart, (size_t)2 * kAmaxWG))) return rc;
     // FP16-exactness and range of T
-    hipLaunchKernelGGL(k_absmax_exact, dim3(1), dim3(1024), 0, c->stream, c->T64, c->N, c->ldt,
-                       c->D, c->scal + 6);
+    hipLaunchKernelGGL(k_absmax_part, dim3(kAmaxWG), dim3(256), 0, c->stream, c->T64, c->N, c->ldt,
+                       c->D, b->amaxPart);
+    DG_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, c->stream, b->amaxPart, kAmaxWG,
+                       c->scal + 6);
     DG_HIP(c, hipGetLastError());
     DG_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, kScalCount * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
@@ -631,6 +742,10 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     const double amax = c->h_scal[6], inexact = c->h_scal[7];
     c->t_exact = (inexact == 0.0 && amax <= 2048.0) ? 1 : 0;
     b->TT = c->t_exact ? 1 : 2;
+    if (b->TT == 2 && KS > 8)
+        return dg_fail(c, DG_EINVAL,
+                       "mixed path: data not exact in FP16 is limited to D <= 128 "
+                       "(use DG_FP64, or FP16-exact data such as binary fingerprints)");
     double tscale = 1.0;
     if (!c->t_exact && amax > 0.0) tscale = ldexp(1.0, 13 - ilogb(amax));
     c->t_scale = (float)tscale;
@@ -640,6 +755,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         (rc = dalloc(c, &c->tt32, (size_t)b->Np)) ||
         (rc = dalloc(c, &b->Y16, (size_t)2 * b->Kp * ld16)) ||
         (rc = dalloc(c, &c->yy32, (size_t)b->Kp)) || (rc = dalloc(c, &b->mscal, 4)) ||
+        (rc = dalloc(c, &b->partLog, (size_t)3 * (b->Np / PT))) ||
         (rc = dalloc(c, &c->pm32, (size_t)b->Np)) || (rc = dalloc(c, &c->pl32, (size_t)b->Np)) ||
         (rc = dalloc(c, &c->partRT32, (size_t)b->NC * c->K * c->D)) ||
         (rc = dalloc(c, &c->partG32, (size_t)b->NC * c->K)))
@@ -664,8 +780,12 @@ int dg_mixed_prepare_t(dg_ctx *c) {
 int dg_mixed_prepare_y(dg_ctx *c) {
     MixedBufs *b = mb(c);
     if (!b || !c->mixed_ready) return dg_fail(c, DG_ESTATE, "mixed buffers not prepared");
-    hipLaunchKernelGGL(k_absmax_exact, dim3(1), dim3(1024), 0, c->stream, c->Y64, (long)c->K,
-                       c->ldt, c->D, c->scal + 6);
+    const int nwg = (int)std::min<long>(kAmaxWG, ((long)c->K * c->D + 4095) / 4096);
+    hipLaunchKernelGGL(k_absmax_part, dim3((unsigned)nwg), dim3(256), 0, c->stream, c->Y64,
+                       (long)c->K, c->ldt, c->D, b->amaxPart);
+    DG_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, c->stream, b->amaxPart, nwg,
+                       c->scal + 6);
     DG_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_make_y16, dim3((unsigned)((b->Kp + 255) / 256)), dim3(256), 0, c->stream,
                        c->Y64, c->ldt, c->K, b->Kp, c->D, c->ldh, c->scal + 6, b->Y16, c->yy32,
@@ -691,7 +811,9 @@ int dg_estep_mixed(dg_ctx *c, int isDA, double temp) {
     hipLaunchKernelGGL(k_reduce32, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, c->stream,
                        c->partRT32, c->partG32, b->NC, c->K, c->D, c->RT, c->G);
     DG_HIP(c, hipGetLastError());
-    if ((rc = dg_reduce_logs(c))) return rc;
+    hipLaunchKernelGGL(k_sum_partlog, dim3(1), dim3(1024), 0, c->stream, b->partLog, b->Np / PT,
+                       c->scal);
+    DG_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_zero_fix, dim3((unsigned)c->D), dim3(256), 0, c->stream, c->scal, c->T64,
                        c->ldt, c->pm, c->N, c->K, c->D, c->RT, c->G);
     DG_HIP(c, hipGetLastError());

@@ -25,8 +25,7 @@ EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload"
 
 
 class DgConfig(C.Structure):
-    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("lambda_", C.c_double),
-                ("y_terms", C.c_int), ("r_terms", C.c_int)]
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("lambda_", C.c_double)]
 
 
 class DgEstepOut(C.Structure):
@@ -96,10 +95,9 @@ def _f64(a):
 class Engine:
     """One DA-GTM problem resident on one GPU."""
 
-    def __init__(self, precision=FP64, lam=0.0, device=0, y_terms=2, r_terms=2):
+    def __init__(self, precision=FP64, lam=0.0, device=0):
         self.L = load()
-        cfg = DgConfig(device=device, precision=precision, lambda_=lam,
-                       y_terms=y_terms, r_terms=r_terms)
+        cfg = DgConfig(device=device, precision=precision, lambda_=lam)
         h = C.c_void_p()
         rc = self.L.dg_create(C.byref(cfg), C.byref(h))
         if rc:

@@ -1,0 +1,393 @@
+// dagtm -- command-line driver of the MI355X DA-GTM engine.
+//
+// Keeps the reference program's options and HDF5 layout (dagtm/dagtm.cxx:
+// options 719-891, fresh start 1104-1159, restart 1161-1213, iteration body
+// 1531-1795, cooling 1909-2206, output 2342-2552, checkpoint 1609-1687,
+// SIGCHLD 2628-2633) and runs every iteration through the C ABI of
+// libdagtm_gpu (include/dagtm_gpu.h) on one GPU.  The MPI process grid of the
+// reference collapses to one process; -z is accepted and ignored.
+#include <getopt.h>
+
+#include <cfloat>
+#include <cmath>
+#include <csignal>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/dagtm_gpu.h"
+#include "dg_h5.hpp"
+#include "dg_init.hpp"
+
+using dgh::H5File;
+using dgh::Mat;
+
+namespace {
+
+volatile std::sig_atomic_t g_force_ckp = 0;
+void on_sigchld(int) { g_force_ckp = 1; }
+
+struct Opts {
+    int L = 3, K = 125, M = 8, maxloop = 0, ckp_every = 100, verbose = 0, device = 0;
+    double alpha = 0.99, delta = 0.1, eps = 1e-6, start_temp = 1.0, lambda = 0.0, s = 2.0;
+    bool isDA = true, need_ctemp = true, checkpointing = true, log_progress = false;
+    bool save_R = false, linear = false, mixed = false, init_only = false, interp = false;
+    unsigned long seed = 0;
+    std::string in = "INPUT.h5", out, restart, ckp_prefix = "ckp.";
+    std::string dT = "T", dX = "X", dFI = "FI", dW = "W", dY = "Y", dR = "R", dbeta = "beta",
+                dM = "M", did = "id", dlb = "lb", dtemp = "temp", diter = "iter", dseed = "seed",
+                dqual = "qual";
+};
+
+void usage(const char *prog) {
+    std::fprintf(stderr,
+                 "usage: %s [OPTIONS]\n"
+                 "-a number    : alpha (default: 0.99)\n"
+                 "-b name      : dataset name for beta (default: beta)\n"
+                 "-B name      : dataset name for label (default: lb)\n"
+                 "-c schedule  : cooling schedule [exp|linear] (default: exp)\n"
+                 "-e number    : precision (convergence threshold, default 1e-6)\n"
+                 "-f dataset   : dataset name for FI (default: FI)\n"
+                 "-h           : show help\n"
+                 "-i filename  : input HDF5 filename (default: INPUT.h5)\n"
+                 "-I           : interpolation: MAP of the -i data under the -r model\n"
+                 "-j number    : maximum number of iterations\n"
+                 "-K number    : number of latent points K (eg, 100, 1k, 1M)\n"
+                 "-L number    : latent dimension\n"
+                 "-m           : EM (toggle deterministic annealing off)\n"
+                 "-M number    : number of basis functions M\n"
+                 "-n           : disable checkpointing\n"
+                 "-o filename  : output HDF5 filename\n"
+                 "-p           : log progress (Temp/LLH/NFree histories)\n"
+                 "-P number    : checkpointing per num. loop (default 100)\n"
+                 "-r filename  : restart from a checkpoint\n"
+                 "-R           : save R\n"
+                 "-s number    : seed\n"
+                 "-S number    : start temp\n"
+                 "-t dataset   : dataset name for T (default: T)\n"
+                 "-v number    : set verbose level\n"
+                 "-w prefix    : checkpointing file prefix (default: ckp.)\n"
+                 "-W dataset   : dataset name for W (default: W)\n"
+                 "-x dataset   : dataset name for X (default: X)\n"
+                 "-z PxQ       : accepted for compatibility; the engine uses one GPU\n"
+                 "--precision fp64|mixed   arithmetic of the E-step (default fp64)\n"
+                 "--lambda x               M-step ridge lambda/beta (default 0)\n"
+                 "--rbf-width s            RBF width in centre spacings (default 2)\n"
+                 "--device n               HIP device (default 0)\n"
+                 "--init-only              write the fresh-start state to -o and exit\n",
+                 prog);
+}
+
+int parse(int argc, char **argv, Opts &o) {
+    static const option longopts[] = {{"precision", required_argument, nullptr, 1},
+                                      {"lambda", required_argument, nullptr, 2},
+                                      {"rbf-width", required_argument, nullptr, 3},
+                                      {"device", required_argument, nullptr, 4},
+                                      {"init-only", no_argument, nullptr, 5},
+                                      {nullptr, 0, nullptr, 0}};
+    int c;
+    opterr = 0;
+    while ((c = getopt_long(argc, argv, "a:b:B:c:e:E:f:g:hHi:Ij:k:K:lL:mM:no:pP:r:Rs:S:t:v:w:W:x:z:",
+                            longopts, nullptr)) != -1) {
+        switch (c) {
+            case 'a': o.alpha = std::atof(optarg); break;
+            case 'b': o.dbeta = optarg; break;
+            case 'B': o.dlb = optarg; break;
+            case 'c':
+                if (!strcasecmp(optarg, "exp")) o.linear = false;
+                else if (!strcasecmp(optarg, "linear")) o.linear = true;
+                else if (!strcasecmp(optarg, "auto")) {
+                    std::fprintf(stderr, "Cooling schedule 'auto' is not supported by this engine\n");
+                    return 1;
+                } else {
+                    std::fprintf(stderr, "Unknown schedule option : %s\n", optarg);
+                    return 1;
+                }
+                break;
+            case 'e': o.eps = std::atof(optarg); break;
+            case 'E': break;                      // eps2: used only by HDA
+            case 'f': o.dFI = optarg; break;
+            case 'g':
+                if (strcasecmp(optarg, "sweep")) {
+                    std::fprintf(stderr, "Only the sweep grid is supported\n");
+                    return 1;
+                }
+                break;
+            case 'h': usage(argv[0]); std::exit(0);
+            case 'H': std::fprintf(stderr, "HDA (-H) is not supported by this engine\n"); return 1;
+            case 'i': o.in = optarg; break;
+            case 'I': o.interp = true; break;
+            case 'j': o.maxloop = std::atoi(optarg); break;
+            case 'k': break;                      // Kmax: used only by HDA
+            case 'K': {
+                float num = 0, unit = 1.0f;
+                char ch = 0;
+                std::sscanf(optarg, "%f%c", &num, &ch);
+                if (ch == 'k' || ch == 'K') unit = 1000;
+                else if (ch == 'm' || ch == 'M') unit = 1000 * 1000;
+                else if (ch == 'g' || ch == 'G') unit = 1000 * 1000 * 1000;
+                o.K = (int)(num * unit);
+            } break;
+            case 'l': break;                      // per-rank log files: one process here
+            case 'L': o.L = std::atoi(optarg); break;
+            case 'm': o.isDA = !o.isDA; break;
+            case 'M': o.M = std::atoi(optarg); break;
+            case 'n': o.checkpointing = false; break;
+            case 'o': o.out = optarg; break;
+            case 'p': o.log_progress = !o.log_progress; break;
+            case 'P': o.ckp_every = std::atoi(optarg); break;
+            case 'r': o.restart = optarg; break;
+            case 'R': o.save_R = !o.save_R; break;
+            case 's': o.seed = (unsigned long)std::atoi(optarg); break;
+            case 'S': o.start_temp = std::atof(optarg); o.need_ctemp = false; break;
+            case 't': o.dT = optarg; break;
+            case 'v': o.verbose = std::atoi(optarg); break;
+            case 'w': o.ckp_prefix = optarg; break;
+            case 'W': o.dW = optarg; break;
+            case 'x': o.dX = optarg; break;
+            case 'z': break;
+            case 1:
+                if (!strcasecmp(optarg, "mixed")) o.mixed = true;
+                else if (!strcasecmp(optarg, "fp64")) o.mixed = false;
+                else { std::fprintf(stderr, "Unknown precision : %s\n", optarg); return 1; }
+                break;
+            case 2: o.lambda = std::atof(optarg); break;
+            case 3: o.s = std::atof(optarg); break;
+            case 4: o.device = std::atoi(optarg); break;
+            case 5: o.init_only = true; break;
+            case '?':
+                if (isprint(optopt)) std::fprintf(stderr, "Unknown option `-%c'.\n", optopt);
+                else std::fprintf(stderr, "Unknown option.\n");
+                return 1;
+            default: return 1;
+        }
+    }
+    return 0;
+}
+
+#define CHECK_DG(call)                                                              \
+    do {                                                                            \
+        int rc_ = (call);                                                           \
+        if (rc_ != DG_OK) {                                                         \
+            std::fprintf(stderr, "%s failed (%d): %s\n", #call, rc_, dg_last_error(ctx)); \
+            dg_destroy(ctx);                                                        \
+            return 2;                                                               \
+        }                                                                           \
+    } while (0)
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Opts o;
+    if (parse(argc, argv, o)) return 1;
+    if (!dgh::have_hdf5()) {
+        std::fprintf(stderr, "this dagtm was built without HDF5 support\n");
+        return 1;
+    }
+    std::signal(SIGCHLD, on_sigchld);
+
+    // ---- input data (dagtm.cxx:919-930, 1076-1102)
+    Mat T;
+    std::vector<int> id, lb;
+    {
+        H5File in = H5File::open_read(o.in);
+        if (!in.ok() || !in.read_matrix(o.dT, T)) {
+            std::fprintf(stderr, "cannot read dataset '%s' from %s\n", o.dT.c_str(), o.in.c_str());
+            return 1;
+        }
+        in.read_ints(o.did, id);
+        in.read_ints(o.dlb, lb);
+    }
+    const long N = T.rows;
+    const int D = (int)T.cols;
+
+    // ---- fresh start or restart
+    Mat X, FI, Y, W;
+    double beta = 0.0, temp = 1.0;
+    int nloop = 0;
+    int K = o.K;
+    if (o.restart.empty()) {
+        std::vector<double> mu, sd;
+        dgh::mean_sd(T, mu, sd);
+        if (!dgh::setup_basis(o.L, o.K, o.M, o.s, X, FI)) {
+            std::fprintf(stderr, "K and M must be perfect L-th powers (K=%d M=%d L=%d)\n", o.K, o.M, o.L);
+            return 1;
+        }
+        if (o.seed == 0) o.seed = (unsigned long)std::time(nullptr);
+        Mat Yrnd = dgh::random_centres(mu, sd, o.K, o.seed);
+        W = dgh::least_squares(FI, Yrnd);
+        beta = (o.K > 1000) ? 0.1 : dgh::initial_beta(Yrnd);
+        Y = Mat(K, D);
+        for (int k = 0; k < K; ++k)
+            for (int d = 0; d < D; ++d) {
+                double s = 0.0;
+                for (long i = 0; i < FI.cols; ++i) s += FI(k, i) * W(i, d);
+                Y(k, d) = s;
+            }
+        if (o.isDA) {
+            if (o.need_ctemp) o.start_temp = dgh::first_critical_temp(T) / std::pow(o.alpha, 10);
+            temp = o.start_temp;
+        }
+    } else {
+        H5File ck = H5File::open_read(o.restart);
+        int seed = 0;
+        if (!ck.ok() || !ck.read_matrix(o.dX, X) || !ck.read_matrix(o.dFI, FI) ||
+            !ck.read_matrix(o.dY, Y) || !ck.read_scalar(o.dbeta, beta) ||
+            !ck.read_scalar(o.dtemp, temp) || !ck.read_scalar(o.diter, nloop)) {
+            std::fprintf(stderr, "cannot read checkpoint %s\n", o.restart.c_str());
+            return 1;
+        }
+        if (ck.read_scalar(o.dseed, seed)) o.seed = (unsigned long)seed;
+        K = (int)FI.rows;
+        o.M = (int)FI.cols - 1;
+        o.L = (int)X.cols;
+        o.start_temp = temp;
+        if (o.maxloop) o.maxloop = nloop + o.maxloop;
+        W = Mat(FI.cols, D);
+    }
+    if (!o.isDA) temp = 1.0;
+
+    if (o.init_only) {
+        if (o.out.empty()) { std::fprintf(stderr, "--init-only needs -o\n"); return 1; }
+        H5File f = H5File::create(o.out);
+        bool ok = f.ok() && f.write_matrix(o.dX, X) && f.write_matrix(o.dFI, FI) &&
+                  f.write_matrix(o.dW, W) && f.write_matrix(o.dY, Y) &&
+                  f.write_scalar(o.dbeta, beta) && f.write_scalar(o.dtemp, temp) &&
+                  f.write_scalar(o.dseed, (int)o.seed);
+        return ok ? 0 : 1;
+    }
+
+    // ---- the engine
+    dg_ctx *ctx = nullptr;
+    dg_config cfg{o.device, o.mixed ? DG_MIXED : DG_FP64, o.lambda};
+    int rc = dg_create(&cfg, &ctx);
+    if (rc != DG_OK) {
+        std::fprintf(stderr, "dg_create failed (%d): no usable HIP device\n", rc);
+        return 2;
+    }
+    CHECK_DG(dg_upload(ctx, T.data(), N, D, D, FI.data(), K, (int)FI.cols, FI.cols, Y.data(), D, beta));
+
+    auto write_ckp = [&](double qual) -> bool {
+        char name[512];
+        std::snprintf(name, sizeof name, "%s%d.h5", o.ckp_prefix.c_str(), nloop);
+        Mat Yc(K, D);
+        double b = 0.0;
+        if (dg_download(ctx, nullptr, 0, Yc.data(), D, &b) != DG_OK) return false;
+        H5File f = H5File::create(name);
+        return f.ok() && f.write_matrix(o.dX, X) && f.write_matrix(o.dFI, FI) &&
+               f.write_matrix(o.dY, Yc) && f.write_scalar(o.dbeta, b) &&
+               f.write_scalar(o.dtemp, temp) && f.write_scalar(o.diter, nloop) &&
+               f.write_scalar(o.dseed, (int)o.seed) && f.write_scalar(o.dqual, qual);
+    };
+
+    // ---- interpolation (dagtm.cxx:1247-1295): responsibilities of the new
+    // data under the model at its beta and temperature, MAP appended to -o
+    if (o.interp) {
+        if (o.restart.empty()) std::fprintf(stderr, "No checkpointing file.. \n");
+        if (o.out.empty()) { std::fprintf(stderr, "-I needs -o\n"); dg_destroy(ctx); return 1; }
+        dg_estep_out eo;
+        CHECK_DG(dg_estep(ctx, o.isDA, temp, &eo));
+        Mat Mo(N, X.cols);
+        CHECK_DG(dg_map(ctx, X.data(), (int)X.cols, X.cols, Mo.data(), X.cols));
+        H5File f = H5File::open_append(o.out);
+        const bool ok = f.ok() && f.write_matrix(o.dM, Mo);
+        dg_destroy(ctx);
+        return ok ? 0 : 1;
+    }
+
+    // ---- annealing loop (dagtm.cxx:1508-2206, EXP / LINEAR / EM)
+    const int QN = 50;
+    std::vector<double> Q(QN);
+    std::vector<double> hT, hL, hF;
+    double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff = 0;
+    bool isLast = false;
+    auto absv = [](double a) { return (a > 0) ? a : -a; };
+    do {
+        if (temp < 1.0) { temp = 1.0; isLast = true; }
+        std::printf("\nT= %.10g\n", temp);
+        std::fill(Q.begin(), Q.end(), DBL_MAX);
+        int qnext = 0, lloop = 0;
+        std::printf(">>> %15s %15s %15s %15s %15s %15s\n", "GSEQ", "SEQ", "K", "TEMP", "NFREE", "LLH");
+        while (true) {
+            dg_estep_out eo;
+            CHECK_DG(dg_estep(ctx, o.isDA, temp, &eo));
+            qual_old = qual;
+            lloop++; nloop++;
+            llh = eo.llh; nfree = eo.nfree;
+            qual = o.isDA ? nfree : llh;
+            diff = qual - qual_old;
+            std::printf("::: %15d %15d %15d %15.7f %15.7f %15.7f %15.7f\n", nloop, lloop, K, temp,
+                        o.isDA ? nfree : -llh, llh, diff);
+            std::fflush(stdout);
+            if (o.log_progress) {
+                hT.push_back(temp); hL.push_back(llh);
+                if (o.isDA) hF.push_back(nfree);
+            }
+            if ((o.checkpointing && o.ckp_every > 0 && nloop % o.ckp_every == 0) || g_force_ckp) {
+                if (!write_ckp(qual)) std::fprintf(stderr, "checkpoint write failed\n");
+                if (g_force_ckp) {
+                    std::fprintf(stderr, "Force checkpointing done\n");
+                    dg_destroy(ctx);
+                    return 0;
+                }
+            }
+            if (absv(diff) < o.eps) break;
+            if (nloop > QN) {
+                double old = 0;
+                for (double q : Q) old += q;
+                old /= QN;
+                Q[qnext] = qual; qnext = (qnext + 1) % QN;
+                double avg = 0;
+                for (double q : Q) avg += q;
+                avg /= QN;
+                if (absv(avg - old) < o.eps) break;
+            }
+            CHECK_DG(dg_mstep(ctx, &beta));
+            if (o.maxloop && nloop >= o.maxloop) { isLast = true; break; }
+        }
+        if (o.isDA && !isLast) {
+            if (o.linear) temp -= o.delta;
+            else temp *= o.alpha;
+        } else {
+            break;
+        }
+    } while (!isLast);
+
+    std::printf("\nSummary: iter %d temp %.10g beta %.10g llh %.10g nfree %.10g\n", nloop, temp,
+                beta, llh, nfree);
+
+    // ---- output (dagtm.cxx:2342-2552)
+    if (!o.out.empty()) {
+        Mat Mo(N, X.cols), Yo(K, D), Wo(FI.cols, D);
+        double b = 0;
+        CHECK_DG(dg_download(ctx, Wo.data(), D, Yo.data(), D, &b));
+        CHECK_DG(dg_map(ctx, X.data(), (int)X.cols, X.cols, Mo.data(), X.cols));
+        H5File f = H5File::create(o.out);
+        bool ok = f.ok() && f.write_matrix(o.dT, T) && f.write_matrix(o.dM, Mo) &&
+                  f.write_matrix(o.dX, X) && f.write_matrix(o.dFI, FI) && f.write_matrix(o.dY, Yo) &&
+                  f.write_matrix(o.dW, Wo) && f.write_scalar(o.dbeta, b) &&
+                  f.write_scalar(o.dtemp, temp) && f.write_scalar(o.diter, nloop) &&
+                  f.write_scalar(o.dseed, (int)o.seed) && f.write_scalar(o.dqual, qual);
+        if (ok && o.save_R) {
+            Mat R(K, N);
+            CHECK_DG(dg_resp_dense(ctx, R.data(), N));
+            ok = f.write_matrix(o.dR, R);
+        }
+        if (ok && !id.empty()) ok = f.write_ints(o.did, id);
+        if (ok && !lb.empty()) ok = f.write_ints(o.dlb, lb);
+        if (ok && o.log_progress) {
+            ok = f.write_vector("Temp", hT) && f.write_vector("LLH", hL);
+            if (ok && o.isDA) ok = f.write_vector("NFree", hF);
+        }
+        if (!ok) {
+            std::fprintf(stderr, "writing %s failed\n", o.out.c_str());
+            dg_destroy(ctx);
+            return 1;
+        }
+    }
+    dg_destroy(ctx);
+    return 0;
+}

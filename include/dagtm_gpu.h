@@ -42,8 +42,6 @@ typedef struct {
     int    device;      /* HIP device ordinal */
     int    precision;   /* enum dg_precision */
     double lambda;      /* M-step ridge, A + (lambda/beta) I; 0 = reference */
-    int    y_terms;     /* DG_MIXED: 16-bit terms per centre coordinate (2|3) */
-    int    r_terms;     /* DG_MIXED: 16-bit terms per responsibility (1|2) */
 } dg_config;
 
 /* Replaces the set-up of dagtm.cxx:893-1068 (MPI_Init, grid, allocation). */
@@ -115,13 +113,18 @@ typedef void (*dg_iter_cb)(void *user, int nloop, int lloop, double temp,
 int dg_anneal(dg_ctx *ctx, const dg_anneal_params *p, dg_anneal_result *res,
               dg_iter_cb cb, void *user);
 
-/* MAP projection M = R^T X (N x L, ldM >= L) with the responsibilities of
- * the last E-step.  Replaces dagtm.cxx:2347-2355. */
+/* MAP projection M = R^T X (N x L, ldM >= L) with the responsibilities at
+ * the current model state and the last E-step's temperature (an E-step is
+ * run first when an M-step has moved the model since).  Computed in FP64
+ * in both precisions.  Replaces dagtm.cxx:2347-2355; unlike the reference
+ * after a maxloop exit (which reuses R o dist, Q4), R here is always the
+ * responsibility matrix. */
 int dg_map(dg_ctx *ctx, const double *X, int L, long ldX, double *M,
            long ldM);
 
-/* Materialises R (K x N, ldR >= N) of the last E-step.  Debug / -R output
- * (dagtm.cxx:2400-2414); intended for small problems. */
+/* Materialises R (K x N, ldR >= N) of the last E-step, in FP64 in both
+ * precisions.  Debug / -R output (dagtm.cxx:2400-2414); intended for small
+ * problems (K x N doubles are allocated on the device for the call). */
 int dg_resp_dense(dg_ctx *ctx, double *R, long ldR);
 
 /* Copies the model back.  Replaces the h5save_byrow of W, Y and beta

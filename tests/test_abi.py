@@ -40,7 +40,7 @@ def test_create_fails_cleanly_without_device():
         pass
     lib = dg.load()
     h = C.c_void_p()
-    cfg = dg.DgConfig(device=0, precision=dg.FP64, lambda_=0.0, y_terms=2, r_terms=2)
+    cfg = dg.DgConfig(device=0, precision=dg.FP64, lambda_=0.0)
     assert lib.dg_create(C.byref(cfg), C.byref(h)) == dg.DG_EHIP
     assert not h.value
 

@@ -139,3 +139,24 @@ def test_anneal_matches_oracle():
     a = np.array(rows_g); b = np.array(rows_o)
     for col in (2, 3, 4, 6):        # temp, nfree, llh, beta
         assert np.allclose(a[:, col], b[:, col], rtol=1e-8, atol=1e-10), col
+
+
+@pytest.mark.parametrize("L,K,M", [(2, 900, 144), (3, 1000, 125)])
+def test_mstep_large_basis_matches_oracle(L, K, M):
+    """M+1 = 145 takes the global-memory factorisation, M+1 = 126 the LDS one."""
+    _say(f"case: M-step K={K} M+1={M + 1}")
+    T = datagen.fingerprints(3000, 40, 3).astype(np.float64)
+    X, FI = datagen.latent_basis(L, K, M, 1.0)
+    rng = np.random.default_rng(1)
+    Y0 = FI @ np.linalg.lstsq(FI, T.mean(0) + 0.2 * rng.standard_normal((K, 40)), rcond=None)[0]
+    lam = 1e-2
+    m = O.Model(T, FI, Y0, 5.0, lam=lam, solver=O.SOLVE_CHOL)
+    m.estep(1, 2.0)
+    m.mstep()
+    with _engine(lam=lam) as e:
+        e.upload(T, FI, Y0, 5.0)
+        e.estep(isDA=True, temp=2.0)
+        beta = e.mstep()
+        W, Y, b = e.download()
+    assert beta == pytest.approx(m.beta, rel=1e-9)
+    assert np.allclose(Y, m.Y, rtol=1e-8, atol=1e-8 * np.abs(m.Y).max())
