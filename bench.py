@@ -14,6 +14,7 @@ K timed steps is the device time.
 Prints one JSON line (rank 0, N = 1 GPU only).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -31,6 +32,26 @@ METRIC = "DA-GTM E+M iter/s at N=1M,K=8k,D=166: mixed-prec vs FP64 GPU, same log
 PEAK_FP16_TFLOPS = 2500.0      # MI355X dense FP16/BF16 MFMA
 TEMP = 2.0
 LAMBDA = 1e-3
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary; None when there is none or when the
+    kernel source changed since it was measured (sha256 recorded beside it)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{kernel}_bench_default.md")
+    src = os.path.join(ROOT, "da-gtm_amd", "csrc", "dg_mixed.hip")
+    try:
+        rec = {}
+        for line in open(path):
+            if ":" in line and line.split(":", 1)[0] in ("traffic_bytes_per_launch", "source_sha256"):
+                k, v = line.split(":", 1)
+                rec[k] = v.split()[0]
+        sha = hashlib.sha256(open(src, "rb").read()).hexdigest()
+        if rec.get("source_sha256") != sha:
+            return None
+        return float(rec["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def build_problem(N, D, K, M, L, seed):
@@ -142,7 +163,8 @@ def main():
         dom, dom_ms, name = "k_e1_mfma", p1_ms, "E-pass 1 (cross term)"
     achieved = flop / (dom_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP16_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4),
+            "traffic": measured_traffic(dom),
             "kernel": dom, "kernel_desc": name, "kernel_ms": round(dom_ms, 4),
             "algorithmic_flop_per_launch": flop}
 

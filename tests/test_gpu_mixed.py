@@ -110,7 +110,11 @@ def test_float_data_vs_fp64(N, D, K, M, L):
             assert l16 == pytest.approx(l64, rel=1e-5)
             assert f16_ == pytest.approx(f64_, rel=1e-5)
             G64, RT64 = e64.stats(); G16, RT16 = e16.stats()
-            assert _rel(G16, G64) < 1e-5 and _rel(RT16, RT64) < 1e-5
+            # Exponents here reach ~100 (log2 units: beta |t|^2 / 2 with |t|^2 ~ 50),
+            # so each FP32 rounding of an exponent moves R by up to 2^-17 ln2 ~ 5e-6;
+            # with N/K = 2.5 points per centre these per-element errors are not
+            # averaged away in G and R T, hence 2e-5 rather than the 1e-5 of binary data.
+            assert _rel(G16, G64) < 2e-5 and _rel(RT16, RT64) < 2e-5
     finally:
         e64.close(); e16.close()
 
