@@ -17,12 +17,12 @@
 // cross-half shuffle finishes the tile.
 //
 // Pass 2 (k_e2_mfma): a workgroup owns 128 centres (32 per wave, centre
-// operand in registers) and one chunk of points, streamed in 32-point blocks.
-// C[n][k] = T Y^T leaves R in the accumulator layout that is exactly the B
-// operand layout of the next product RT^T[d][k] += sum_n T^T[d][n] R[n][k], so
-// R goes from the exponential straight into the second MFMA as an FP16
-// hi/lo pair without touching LDS.  Each chunk writes FP32 partials that a
-// deterministic FP64 pass sums.
+// operand in registers) and one chunk of points, streamed in 32-point blocks,
+// two blocks in flight per wave.  C[n][k] = T Y^T leaves R in the accumulator
+// layout that is exactly the B operand layout of the next product
+// RT^T[d][k] += sum_n T^T[d][n] R[n][k], so R goes from the exponential
+// straight into the second MFMA as an FP16 hi/lo pair without touching LDS.
+// Each chunk writes FP32 partials that a deterministic FP64 pass sums.
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
@@ -33,6 +33,7 @@ namespace {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -67,8 +68,10 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp,
     const double *__restrict__ mscal, float beta, float it, int two, int isDA, double logK,
-    float *__restrict__ pB, float *__restrict__ pThr, double *__restrict__ pm,
-    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
+    float *__restrict__ pB, float *__restrict__ pThr, float *__restrict__ blkThr,
+    double *__restrict__ pm, double *__restrict__ logg, double *__restrict__ logh,
+    double *__restrict__ partLog) {
+    static_assert(PT == 4 * BP, "one pass-1 wave covers one pass-2 block");
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;
     constexpr int ROW8 = ld / 8;                 // h8 chunks per row
@@ -188,12 +191,14 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     const float S1o = __shfl_xor(S1, 32), C1o = __shfl_xor(C1, 32);
     const float S2o = __shfl_xor(S2, 32), C2o = __shfl_xor(C2, 32);
     double lg = 0.0, lh = 0.0, zc = 0.0;      // this lane's share of the workgroup sums
+    // Padding points and zero columns get pB = -inf: every pass-2 exponent of
+    // theirs is -inf, so R = 0 there without any masking (a fix-up adds 1/K
+    // for zero columns afterwards).
+    float pbv = -INFINITY, ptv = -INFINITY;
     if (h == 0) {
         const double s1 = ((double)S1 - C1) + ((double)S1o - C1o);
         const double s2 = two ? ((double)S2 - C2) + ((double)S2o - C2o) : s1;
         if (n >= N) {
-            pB[n] = 0.f;
-            pThr[n] = INFINITY;
         } else if (!(s1 > 0.0)) {          // zero column (mpicomm.c:382-390)
             lg = isDA ? -INFINITY : logK;
             lh = logK;
@@ -201,8 +206,6 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             pm[n] = -INFINITY;
             logg[n] = lg;
             logh[n] = lh;
-            pB[n] = 0.f;
-            pThr[n] = INFINITY;            // pass 2 skips it; a fix-up adds 1/K
         } else {
             const double L1 = log2(s1), L2 = log2(s2);
             const double ma = (double)m + (double)bn;        // the point's max exponent
@@ -211,10 +214,18 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             pm[n] = ma;
             logg[n] = lg;
             logh[n] = lh;
-            pB[n] = (float)(-(double)it * m - L2 + kRLog2);
-            pThr[n] = (float)((double)it * ((double)thr - m) - L2 + kRLog2);
+            pbv = (float)(-(double)it * m - L2 + kRLog2);
+            ptv = (float)((double)it * ((double)thr - m) - L2 + kRLog2);
         }
+        pB[n] = pbv;
+        pThr[n] = ptv;
     }
+    // This wave's 32 points are one pass-2 block: its largest mask threshold
+    // tells pass 2 whether the block needs the underflow mask at all.
+    float bt = ptv;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bt = fmaxf(bt, __shfl_xor(bt, o));
+    if (l == 0) blkThr[(long)blockIdx.x * (PT / BP) + w] = bt;
     // workgroup partials of sum log g, sum log h and the zero-column count
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) {
@@ -255,19 +266,30 @@ __global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__
 }
 
 // ---------------------------------------------------------------- pass 2
-// Per 32-point block: LDS-staged data (double-buffered, the next block
-// prefetched into registers), cross-term MFMAs, the exponential epilogue and
-// the R T MFMAs run in one straight-line region, so every read of an MFMA
-// result stays in the basic block of the MFMA that produced it.  The 1-D
-// grid is mapped so that the workgroups the hardware places on one XCD
-// (linear id mod 8) share a point chunk and walk all its centre groups,
-// keeping that chunk's data in the XCD's L2.
-template <int KS, int TT>
-__global__ __launch_bounds__(NTH) void k_e2_mfma(
+// A workgroup owns 128 centres (32 per wave, both FP16 terms of Y held in
+// registers as the B operand of the cross term) and one chunk of points,
+// streamed in 32-point blocks through a three-slot LDS ring (the data rows
+// for the cross term, the transposed rows for R T, pB and the mask
+// thresholds).  Two blocks are in flight: while the cross-term MFMAs of block
+// b+1 run, the wave turns block b's exponents into R (v_exp_f32, FP16 hi/lo
+// pairs); while block b's R T MFMAs run, it forms block b+1's exponents.  Only
+// VALU results (the exponents) cross the loop's back edge; every read of an
+// MFMA result sits in the basic block of the MFMA that produced it.
+//
+// T16t stores the points of every 16-point group in the order
+// 0-3, 8-11, 4-7, 12-15, so that one 16-byte LDS read gives a lane the eight
+// points its R operand holds (the accumulator layout of the cross term).
+// With GM, T16t row D is 1 for real points, and the R T product yields G in
+// that row.  The 1-D grid is mapped so that the workgroups the hardware places
+// on one XCD (linear id mod 8) share a point chunk and walk all its centre
+// groups, keeping that chunk's data in the XCD's L2.
+template <int KS, int TT, bool GM>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_e2_mfma(
     const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it,
-    const float *__restrict__ pB, const float *__restrict__ pThr, long chunk, int KG,
+    const float *__restrict__ pB, const float *__restrict__ pThr,
+    const float *__restrict__ blkThr, long chunk, int KG,
     float *__restrict__ partRT, float *__restrict__ partG) {
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;
@@ -279,10 +301,11 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
     constexpr int CHB = TT * DR * (BP / 8);
     constexpr int PA = (CHA + NTH - 1) / NTH;
     constexpr int PB_ = (CHB + NTH - 1) / NTH;
-    __shared__ __attribute__((aligned(16))) _Float16 Ts[2][TT][BP][LR];
-    __shared__ __attribute__((aligned(16))) _Float16 Tts[2][TT][DR][TR];
-    __shared__ __attribute__((aligned(16))) float pBs[2][BP];
-    __shared__ __attribute__((aligned(16))) float pTs[2][BP];
+    constexpr int NS = 3;                          // LDS ring slots
+    __shared__ __attribute__((aligned(16))) _Float16 Ts[NS][TT][BP][LR];
+    __shared__ __attribute__((aligned(16))) _Float16 Tts[NS][TT][DR][TR];
+    __shared__ __attribute__((aligned(16))) float pBs[NS][BP];
+    __shared__ __attribute__((aligned(16))) float pTs[NS][BP];
     const int L = blockIdx.x;
     const int xs = L & 7, sl = L >> 3;
     const int kg = sl % KG;
@@ -304,13 +327,17 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
     for (int b = 0; b < DB; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
-    float G = 0.f, Gc = 0.f;
+    float G = 0.f, Gc = 0.f;                       // VALU row sums when !GM
     const long nb = cidx * chunk;
     const long ne = (nb + chunk < Np) ? nb + chunk : Np;
     const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
+
     h8 preA[PA], preB[PB_];
     float pp = 0.f;
-    auto gload = [&](long n0) {
+    // global -> registers for block q (clamped into the chunk: a repeated
+    // block is loaded but never used)
+    auto gload = [&](int q) {
+        const long n0 = nb + (long)(q < nblocks ? q : nblocks - 1) * BP;
 #pragma unroll
         for (int p = 0; p < PA; ++p) {
             const int c = tid + p * NTH;
@@ -332,14 +359,14 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
         if (tid < BP) pp = pB[n0 + tid];
         else if (tid < 2 * BP) pp = pThr[n0 + tid - BP];
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int s) {
 #pragma unroll
         for (int p = 0; p < PA; ++p) {
             const int c = tid + p * NTH;
             if (c < CHA) {
                 const int term = c / (BP * ROW8), rem = c % (BP * ROW8);
                 const int row = rem / ROW8, col = rem % ROW8;
-                *(h8 *)&Ts[buf][term][row][col * 8] = preA[p];
+                *(h8 *)&Ts[s][term][row][col * 8] = preA[p];
             }
         }
 #pragma unroll
@@ -348,87 +375,108 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
             if (c < CHB) {
                 const int term = c / (DR * (BP / 8)), rem = c % (DR * (BP / 8));
                 const int row = rem / (BP / 8), col = rem % (BP / 8);
-                *(h8 *)&Tts[buf][term][row][col * 8] = preB[p];
+                *(h8 *)&Tts[s][term][row][col * 8] = preB[p];
             }
         }
-        if (tid < BP) pBs[buf][tid] = pp;
-        else if (tid < 2 * BP) pTs[buf][tid - BP] = pp;
+        if (tid < BP) pBs[s][tid] = pp;
+        else if (tid < 2 * BP) pTs[s][tid - BP] = pp;
     };
-    if (nblocks > 0) { gload(nb); lstore(0); }
-    __syncthreads();
-    for (int b = 0; b < nblocks; ++b) {
-        const int buf = b & 1;
-        if (b + 1 < nblocks) gload(nb + (long)(b + 1) * BP);
+    // cross term of the block in slot s -> exponents (base 2, R * 2^14 scale)
+    auto cross_exp = [&](int s, float (&e)[16]) {
         f16v c;
 #pragma unroll
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            const h8 a0 = *(const h8 *)&Ts[buf][0][j][16 * ks + 8 * h];
+            const h8 a0 = *(const h8 *)&Ts[s][0][j][16 * ks + 8 * h];
             c = mfma(a0, yb[0][ks], c);
             c = mfma(a0, yb[1][ks], c);
             if (TT == 2) {
-                const h8 a1 = *(const h8 *)&Ts[buf][TT - 1][j][16 * ks + 8 * h];
+                const h8 a1 = *(const h8 *)&Ts[s][TT - 1][j][16 * ks + 8 * h];
                 c = mfma(a1, yb[0][ks], c);
             }
         }
-        // Exponents first, in the basic block of the MFMAs that produced c.
-        // Elements below a point's threshold carry less than 2^-64 when the
-        // threshold itself is below -64, so the underflow mask (a VALU-only
-        // block) runs only for blocks that need it.
-        float e[16];
-        float ptmax = -INFINITY;
 #pragma unroll
         for (int m4 = 0; m4 < 4; ++m4) {
-            const f4v pb = *(const f4v *)&pBs[buf][8 * m4 + 4 * h];
-            const f4v pt = *(const f4v *)&pTs[buf][8 * m4 + 4 * h];
-            ptmax = fmaxf(ptmax, fmaxf(fmaxf(pt[0], pt[1]), fmaxf(pt[2], pt[3])));
+            const f4v pb = *(const f4v *)&pBs[s][8 * m4 + 4 * h];
 #pragma unroll
             for (int q = 0; q < 4; ++q) e[4 * m4 + q] = fmaf(itc1, c[4 * m4 + q], aKi) + pb[q];
         }
-        if (!__all(ptmax < -kMaskMargin)) {
+    };
+    // exp-before-pow underflow: elements below their point's threshold are 0
+    auto mask = [&](int s, float (&e)[16]) {
 #pragma unroll
-            for (int m4 = 0; m4 < 4; ++m4) {
-                const f4v pt = *(const f4v *)&pTs[buf][8 * m4 + 4 * h];
+        for (int m4 = 0; m4 < 4; ++m4) {
+            const f4v pt = *(const f4v *)&pTs[s][8 * m4 + 4 * h];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    e[4 * m4 + q] = (e[4 * m4 + q] < pt[q]) ? -INFINITY : e[4 * m4 + q];
-            }
+            for (int q = 0; q < 4; ++q)
+                e[4 * m4 + q] = (e[4 * m4 + q] < pt[q]) ? -INFINITY : e[4 * m4 + q];
         }
-        h8 rhi[2], rlo[2];
+    };
+    // R = 2^e as FP16 hi/lo pairs in the B-operand order of the R T product
+    auto resp = [&](const float (&e)[16], h8 (&rhi)[2], h8 (&rlo)[2]) {
         float gs = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const float R = ex2(e[r]);
-            gs += R;
-            const _Float16 hi = (_Float16)R;
-            const _Float16 lo = (_Float16)(R - (float)hi);
-            rhi[r >> 3][r & 7] = hi;
-            rlo[r >> 3][r & 7] = lo;
+        for (int r = 0; r < 16; r += 2) {
+            const float R0 = ex2(e[r]), R1 = ex2(e[r + 1]);
+            if (!GM) gs += R0 + R1;
+            const h2 hp = {(_Float16)R0, (_Float16)R1};
+            const h2 lp = {(_Float16)(R0 - (float)hp[0]), (_Float16)(R1 - (float)hp[1])};
+            rhi[r >> 3][r & 7] = hp[0]; rhi[r >> 3][(r & 7) + 1] = hp[1];
+            rlo[r >> 3][r & 7] = lp[0]; rlo[r >> 3][(r & 7) + 1] = lp[1];
         }
-        kahan(G, Gc, gs);
+        if (!GM) kahan(G, Gc, gs);
+    };
+    auto rt = [&](int s, const h8 (&rhi)[2], const h8 (&rlo)[2]) {
 #pragma unroll
         for (int db = 0; db < DB; ++db) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const h4 u0 = *(const h4 *)&Tts[buf][0][32 * db + j][16 * s + 4 * h];
-                const h4 u1 = *(const h4 *)&Tts[buf][0][32 * db + j][16 * s + 8 + 4 * h];
-                const h8 a0 = {u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
-                acc[db] = mfma(a0, rhi[s], acc[db]);
-                acc[db] = mfma(a0, rlo[s], acc[db]);
+            for (int sh = 0; sh < 2; ++sh) {
+                const h8 a0 = *(const h8 *)&Tts[s][0][32 * db + j][16 * sh + 8 * h];
+                acc[db] = mfma(a0, rhi[sh], acc[db]);
+                acc[db] = mfma(a0, rlo[sh], acc[db]);
                 if (TT == 2) {
-                    const h4 v0 = *(const h4 *)&Tts[buf][TT - 1][32 * db + j][16 * s + 4 * h];
-                    const h4 v1 = *(const h4 *)&Tts[buf][TT - 1][32 * db + j][16 * s + 8 + 4 * h];
-                    const h8 a1 = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
-                    acc[db] = mfma(a1, rhi[s], acc[db]);
+                    const h8 a1 = *(const h8 *)&Tts[s][TT - 1][32 * db + j][16 * sh + 8 * h];
+                    acc[db] = mfma(a1, rhi[sh], acc[db]);
                 }
             }
         }
-        if (b + 1 < nblocks) lstore(buf ^ 1);
+    };
+    const long blk0 = nb / BP;                     // index of the chunk's first block
+
+    if (nblocks > 0) {
+        gload(0); lstore(0);
+        gload(1); lstore(1);
+        gload(2);
         __syncthreads();
+        float e[16];
+        cross_exp(0, e);
+        if (blkThr[blk0] > -kMaskMargin) mask(0, e);
+        for (int b = 0; b + 1 < nblocks; ++b) {
+            const int s0 = b % NS, s1 = (b + 1) % NS, s2 = (b + 2) % NS;
+            h8 rhi[2], rlo[2];
+            resp(e, rhi, rlo);                     // block b: R from its exponents
+            float en[16];
+            cross_exp(s1, en);                     // block b+1: cross term
+            rt(s0, rhi, rlo);                      // block b: R T
+            lstore(s2);                            // block b+2 into the free slot
+            gload(b + 3);
+            if (blkThr[blk0 + b + 1] > -kMaskMargin) mask(s1, en);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) e[r] = en[r];
+            __syncthreads();
+        }
+        h8 rhi[2], rlo[2];
+        resp(e, rhi, rlo);
+        rt((nblocks - 1) % NS, rhi, rlo);
     }
+    // the accumulators are read after the loop's exit branch: keep the
+    // scheduler from moving those reads up and give the MFMA pipe time to drain
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
     const float inv = (float)(1.0 / (16384.0 * mscal[1]));
     const float ginv = 1.0f / 16384.0f;
+    float gval = 0.f;
     if (k < K) {
 #pragma unroll
         for (int db = 0; db < DB; ++db)
@@ -436,10 +484,18 @@ __global__ __launch_bounds__(NTH) void k_e2_mfma(
             for (int r = 0; r < 16; ++r) {
                 const int d = 32 * db + 8 * (r >> 2) + 4 * h + (r & 3);
                 if (d < D) partRT[(cidx * D + d) * K + k] = acc[db][r] * inv;
+                if (GM && d == D) gval = acc[db][r];
             }
     }
-    const float Go = __shfl_xor(G, 32), Gco = __shfl_xor(Gc, 32);
-    if (h == 0 && k < K) partG[cidx * K + k] = (((G - Gc) + (Go - Gco))) * ginv;
+    if (GM) {
+        // row D lives in one half of the wave: take it from whichever holds it
+        const float go = __shfl_xor(gval, 32);
+        const bool mine = (((D & 31) >> 2) & 1) == h;
+        if (h == 0 && k < K) partG[cidx * K + k] = (mine ? gval : go) * ginv;
+    } else {
+        const float Go = __shfl_xor(G, 32), Gco = __shfl_xor(Gc, 32);
+        if (h == 0 && k < K) partG[cidx * K + k] = (((G - Gc) + (Go - Gco))) * ginv;
+    }
 }
 
 // ---------------------------------------------------------------- helpers
@@ -558,9 +614,13 @@ __global__ void k_make_t16(const double *__restrict__ T64, int ldt, long N, long
     tt32[n] = (float)ss;
 }
 
-// T16t[t][d][n] = T16[t][n][d] for d < DR (rows >= ld16 zero), 64 x 64 tiles.
-__global__ void k_transpose16(const _Float16 *__restrict__ T16, long Np, int ld16, int DR,
-                              _Float16 *__restrict__ T16t) {
+// T16t[t][d][pos(n)] = T16[t][n][d] for d < DR (rows >= ld16 zero), with the
+// points of each 16-point group stored in the order 0-3, 8-11, 4-7, 12-15
+// (pos swaps the two middle quads) so that a lane's eight R-operand points
+// are one 16-byte run.  With ones_row >= 0, row ones_row of term 0 is 1 for
+// real points (n < N) and 0 for padding.  64 x 64 tiles.
+__global__ void k_transpose16(const _Float16 *__restrict__ T16, long N, long Np, int ld16, int DR,
+                              int ones_row, _Float16 *__restrict__ T16t) {
     __shared__ _Float16 tile[64][65];
     const long n0 = (long)blockIdx.x * 64;
     const int d0 = blockIdx.y * 64;
@@ -569,14 +629,18 @@ __global__ void k_transpose16(const _Float16 *__restrict__ T16, long Np, int ld1
         const int r = e / 64, c = e % 64;           // r: point, c: dim
         const long n = n0 + r;
         const int d = d0 + c;
-        tile[r][c] = (n < Np && d < ld16) ? T16[((long)t * Np + n) * ld16 + d] : (_Float16)0.0f;
+        _Float16 v = (n < Np && d < ld16) ? T16[((long)t * Np + n) * ld16 + d] : (_Float16)0.0f;
+        if (d == ones_row && t == 0) v = (n < N) ? (_Float16)1.0f : (_Float16)0.0f;
+        tile[r][c] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
-        const int r = e / 64, c = e % 64;           // r: dim, c: point
+        const int r = e / 64, c = e % 64;           // r: dim, c: stored position
+        const int q = (c >> 2) & 3;
+        const int src = (c & ~15) | ((((q & 1) << 1) | (q >> 1)) << 2) | (c & 3);
         const int d = d0 + r;
         const long n = n0 + c;
-        if (d < DR && n < Np) T16t[((long)t * DR + d) * Np + n] = tile[c][r];
+        if (d < DR && n < Np) T16t[((long)t * DR + d) * Np + n] = tile[src][r];
     }
 }
 
@@ -618,6 +682,7 @@ struct MixedBufs {
     _Float16 *Y16 = nullptr;
     double *mscal = nullptr;
     double *partLog = nullptr;    // 3 per pass-1 workgroup
+    float *blkThr = nullptr;      // per 32-point block: largest mask threshold
     double *amaxPart = nullptr;   // 2 x kAmaxWG
     long Np = 0;
     int Kp = 0, KS = 0, TT = 1, NC = 1;
@@ -630,16 +695,24 @@ void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int is
                   double logK) {
     hipLaunchKernelGGL((k_e1_mfma<KS, TT>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0, c->stream,
                        (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16, c->yy32, c->K,
-                       b->Kp, b->mscal, beta, it, two, isDA, logK, c->pm32, c->pl32, c->pm,
-                       c->logg, c->logh, b->partLog);
+                       b->Kp, b->mscal, beta, it, two, isDA, logK, c->pm32, c->pl32, b->blkThr,
+                       c->pm, c->logg, c->logh, b->partLog);
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
     const long blocks = b->Np / BP;
     const long chunk = ((blocks + b->NC - 1) / b->NC) * BP;
     const int KG = b->Kp / CT;
-    hipLaunchKernelGGL((k_e2_mfma<KS, TT>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0, c->stream,
-                       (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32, c->K, b->Kp,
-                       c->D, b->mscal, beta, it, c->pm32, c->pl32, chunk, KG, c->partRT32,
-                       c->partG32);
+    // G comes out of the R T product when the padded d range has a spare row
+    constexpr int DR = 32 * ((16 * KS + 31) / 32);
+    if (DR > c->D)
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
+                           c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
+                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pm32, c->pl32, b->blkThr,
+                           chunk, KG, c->partRT32, c->partG32);
+    else
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
+                           c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
+                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pm32, c->pl32, b->blkThr,
+                           chunk, KG, c->partRT32, c->partG32);
 }
 
 template <int TT>
@@ -653,7 +726,7 @@ int dispatch_ks(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int isDA
         case 8: launch_mixed<8, TT>(c, b, beta, it, two, isDA, logK); break;
         case 11:
         case 12:
-            // the split data's two pass-2 LDS rings exceed 160 KB above D = 128
+            // the split data's pass-2 LDS ring (three slots, two terms) exceeds 160 KB above D = 128
             if constexpr (TT == 1) {
                 if (b->KS == 11) launch_mixed<11, 1>(c, b, beta, it, two, isDA, logK);
                 else launch_mixed<12, 1>(c, b, beta, it, two, isDA, logK);
@@ -688,6 +761,7 @@ void dg_mixed_free(dg_ctx *c) {
         if (b->Y16) (void)hipFree(b->Y16);
         if (b->mscal) (void)hipFree(b->mscal);
         if (b->partLog) (void)hipFree(b->partLog);
+        if (b->blkThr) (void)hipFree(b->blkThr);
         if (b->amaxPart) (void)hipFree(b->amaxPart);
         delete b;
     }
@@ -756,6 +830,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         (rc = dalloc(c, &b->Y16, (size_t)2 * b->Kp * ld16)) ||
         (rc = dalloc(c, &c->yy32, (size_t)b->Kp)) || (rc = dalloc(c, &b->mscal, 4)) ||
         (rc = dalloc(c, &b->partLog, (size_t)3 * (b->Np / PT))) ||
+        (rc = dalloc(c, &b->blkThr, (size_t)(b->Np / BP))) ||
         (rc = dalloc(c, &c->pm32, (size_t)b->Np)) || (rc = dalloc(c, &c->pl32, (size_t)b->Np)) ||
         (rc = dalloc(c, &c->partRT32, (size_t)b->NC * c->K * c->D)) ||
         (rc = dalloc(c, &c->partG32, (size_t)b->NC * c->K)))
@@ -769,8 +844,8 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     hipLaunchKernelGGL(k_transpose16,
                        dim3((unsigned)((b->Np + 63) / 64), (unsigned)((DR + 63) / 64),
                             (unsigned)b->TT),
-                       dim3(256), 0, c->stream, (const _Float16 *)c->T16, b->Np, ld16, DR,
-                       b->T16t);
+                       dim3(256), 0, c->stream, (const _Float16 *)c->T16, c->N, b->Np, ld16, DR,
+                       DR > c->D ? c->D : -1, b->T16t);
     DG_HIP(c, hipGetLastError());
     DG_HIP(c, hipStreamSynchronize(c->stream));
     c->mixed_ready = 1;
