@@ -65,8 +65,8 @@ struct dg_ctx {
     void *T16 = nullptr;           // TT x Np x ldh   FP16 terms of T
     float *tt32 = nullptr;         // Np   |t'|^2
     float *yy32 = nullptr;         // Kp   |y'|^2
-    float *pm32 = nullptr;         // Np   pass-2 exponent offset per point
-    float *pl32 = nullptr;         // Np   pass-2 underflow threshold per point
+    float *pP32 = nullptr;         // 2 Np: per 32-point block, 32 pass-2 exponent
+                                   // offsets then 32 underflow thresholds
     float *partRT32 = nullptr;     // NC x D x K
     float *partG32 = nullptr;      // NC x K
 

@@ -12,9 +12,9 @@
 //
 // Pass 1 (k_e1_mfma): a workgroup owns 128 points (32 per wave, their data
 // operand held in registers) and streams all centres in 32-centre tiles
-// through double-buffered LDS.  C[k][n] = Y T^T puts one point per lane
-// column, so each lane reduces its 16 centre rows in registers and one
-// cross-half shuffle finishes the tile.
+// through an LDS ring, two tiles in flight per wave.  C[k][n] = Y T^T puts
+// one point per lane column, so each lane reduces its 16 centre rows in
+// registers and one cross-half exchange finishes the tile.
 //
 // Pass 2 (k_e2_mfma): a workgroup owns 128 centres (32 per wave, centre
 // operand in registers) and one chunk of points, streamed in 32-point blocks,
@@ -34,6 +34,7 @@ namespace {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -63,25 +64,45 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 }
 
 // ---------------------------------------------------------------- pass 1
-template <int KS, int TT>
+// A workgroup owns 128 points (32 per wave; their FP16 data held in
+// registers as the B operand) and streams all centres in 32-centre tiles
+// through a two-slot LDS ring filled by LDS-DMA (both FP16 terms of the
+// tile's centres and their exponent terms aK = -beta log2(e) |y|^2 / 2).
+// C[k][n] = Y T^T puts one point per lane column, so each lane reduces its 16
+// centre rows in registers and one cross-half exchange finishes the tile.
+// Two tiles are in flight per wave, in a fixed issue order held by
+// sched_barrier: each cross-term MFMA of tile t+1 is followed by the
+// exponentials and sums of one centre row of tile t (or a DMA load of tile
+// t+2); tile t+1's exponents, maximum and rescaling then follow in the basic
+// block of its MFMAs.  Only VALU results (the exponents) cross the loop's back
+// edge.
+template <int KS, int TT, bool TWO>
 __global__ __launch_bounds__(NTH) void k_e1_mfma(
     const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
-    const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp,
-    const double *__restrict__ mscal, float beta, float it, int two, int isDA, double logK,
-    float *__restrict__ pB, float *__restrict__ pThr, float *__restrict__ blkThr,
-    double *__restrict__ pm, double *__restrict__ logg, double *__restrict__ logh,
-    double *__restrict__ partLog) {
+    const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
+    const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
+    float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
+    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
     static_assert(PT == 4 * BP, "one pass-1 wave covers one pass-2 block");
     constexpr int ld = 16 * KS;
-    constexpr int LR = ld + PAD;
-    constexpr int ROW8 = ld / 8;                 // h8 chunks per row
-    constexpr int CH = 2 * KT * ROW8;            // h8 chunks per tile (two terms)
-    constexpr int PER = (CH + NTH - 1) / NTH;
-    __shared__ __attribute__((aligned(16))) _Float16 Ys[2][2][KT][LR];
-    __shared__ float aKs[2][KT];
+    constexpr int LR = ld + PAD;                   // centre row in LDS (halves), padding last
+    constexpr int RC = LR / 8;                     // 16-byte chunks per row
+    constexpr int NIA = (2 * KT * RC + 63) / 64;   // 1 KB LDS-DMA loads: both Y terms,
+    constexpr int NI = NIA + 1;                    // + the tile's aK
+    constexpr int NIW = (NI + 3) / 4;              // per wave
+    constexpr int SLOT = 4 * NIW * 512;            // halves per ring slot
+    constexpr int OFF_AK = NIA * 512;
+    // Iteration t reads only the slot of tile t+1 (tile t's exponents are in
+    // registers), so the DMA of tile t+2 goes to the slot tile t came from:
+    // two slots suffice.
+    constexpr int NS = 2;
+    constexpr int NX = (TT + 1) * KS;              // cross-term MFMAs per tile
+    static_assert(NS * SLOT * 2 <= 80 * 1024, "two workgroups per CU");
+    __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
     __shared__ double wsum[3][4];
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
     const long n = (long)blockIdx.x * PT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
@@ -98,59 +119,64 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         for (int ks = 0; ks < KS; ++ks)
             tb[t][ks] = *(const h8 *)(T16 + ((long)t * Np + n) * ld + 16 * ks + 8 * h);
 
-    h8 pre[PER];
-    float apre = 0.f;
-    auto gload = [&](int k0) {
+    // DMA: load I of a tile fills chunks 64 I .. 64 I + 63 of the slot in lane
+    // order ([Y hi rows | Y lo rows | aK]); wave w issues loads w, w + 4, ...
+    // Padding chunks take a copy of the load's first source chunk.
+    unsigned voff[NIW];
+    const char *dbase[NIW];
+    unsigned dstep[NIW];
 #pragma unroll
-        for (int p = 0; p < PER; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CH) {
-                const int term = c / (KT * ROW8), rem = c % (KT * ROW8);
-                const int row = rem / ROW8, col = rem % ROW8;
-                pre[p] = *(const h8 *)(Y16 + ((long)term * Kp + k0 + row) * ld + col * 8);
-            }
+    for (int i = 0; i < NIW; ++i) {
+        const int I = wu + 4 * i;
+        unsigned o = 0;
+        if (I < NIA) {
+            const int g = I * 64 + l;
+            const int t = g / (KT * RC), rem = g % (KT * RC), row = rem / RC, col = rem % RC;
+            if (t < 2 && col < ld / 8) o = (unsigned)((((long)t * Kp + row) * ld + col * 8) * 2);
+        } else if (I == NIA) {
+            if (l < KT / 4) o = (unsigned)(l * 16);
         }
-        if (tid < KT) apre = (k0 + tid < K) ? cb * yy32[k0 + tid] : -INFINITY;
+        voff[i] = o;
+        dbase[i] = (I < NIA) ? (const char *)Y16 : (const char *)aK32;
+        dstep[i] = (I < NIA) ? (unsigned)(KT * ld * 2) : (unsigned)(KT * 4);
+    }
+    const int ntiles = Kp / KT;
+    auto dma = [&](int q, int s, int i) {
+        const unsigned qc = (unsigned)(q < ntiles ? q : ntiles - 1);
+        const int I = wu + 4 * i;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
+            (__attribute__((address_space(3))) void *)&smem[s][I * 512], 16, 0, 0);
     };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int p = 0; p < PER; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CH) {
-                const int term = c / (KT * ROW8), rem = c % (KT * ROW8);
-                const int row = rem / ROW8, col = rem % ROW8;
-                *(h8 *)&Ys[buf][term][row][col * 8] = pre[p];
-            }
-        }
-        if (tid < KT) aKs[buf][tid] = apre;
+    auto ldY = [&](int s, int t, int ks) {         // A operand: centre row j, term t
+        return *(const h8 *)&smem[s][(t * KT + j) * LR + 16 * ks + 8 * h];
+    };
+    auto ldK = [&](int s, int m4) {                // aK of rows 8 m4 + 4 h .. + 3
+        return *(const f4v *)&smem[s][OFF_AK + 2 * (8 * m4 + 4 * h)];
+    };
+    // cross-term MFMA number m (m = (TT + 1) ks + which) of the tile whose
+    // Y rows are a0 (hi) / a1 (lo)
+    auto xmfma = [&](int m, const h8 &a0, const h8 &a1, f16v &c) {
+        const int ks = m / (TT + 1), wq = m % (TT + 1);
+        if (wq == 0) c = mfma(a0, tb[0][ks], c);
+        else if (wq == 1) c = mfma(a1, tb[0][ks], c);
+        else c = mfma(a0, tb[TT - 1][ks], c);
     };
 
     float m = thr, S1 = 0.f, C1 = 0.f, S2 = 0.f, C2 = 0.f;
-    const int ntiles = Kp / KT;
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) gload((t + 1) * KT);
-        f16v acc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const h8 a0 = *(const h8 *)&Ys[buf][0][j][16 * ks + 8 * h];
-            const h8 a1 = *(const h8 *)&Ys[buf][1][j][16 * ks + 8 * h];
-            acc = mfma(a0, tb[0][ks], acc);
-            acc = mfma(a1, tb[0][ks], acc);
-            if (TT == 2) acc = mfma(a0, tb[TT - 1][ks], acc);
-        }
-        float z[16];
+    float t1 = 0.f, t2 = 0.f, nmit = 0.f;
+    // exponents of a tile from its accumulator, the tile maximum, the
+    // underflow mask when it can matter, and the online rescaling
+    auto tile_exp = [&](int s, const f16v &c, float (&z)[16]) {
         float zt = -INFINITY;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-            z[r] = fmaf(c1, acc[r], aKs[buf][i]);
-            zt = fmaxf(zt, z[r]);
+        for (int m4 = 0; m4 < 4; ++m4) {
+            const f4v ak = ldK(s, m4);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                z[4 * m4 + q] = fmaf(c1, c[4 * m4 + q], ak[q]);
+                zt = fmaxf(zt, z[4 * m4 + q]);
+            }
         }
         zt = fmaxf(zt, __shfl_xor(zt, 32));
         // The underflow mask only matters when a masked element could weigh
@@ -169,24 +195,80 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             const float dm = m - zt;
             const float s1 = ex2(dm);
             S1 *= s1; C1 *= s1;
-            if (two) {
+            if (TWO) {
                 const float s2 = ex2(dm * it);
                 S2 *= s2; C2 *= s2;
             }
             m = zt;
         }
-        float t1 = 0.f, t2 = 0.f;
-        const float nmit = -m * it;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            t1 += ex2(z[r] - m);
-            if (two) t2 += ex2(fmaf(z[r], it, nmit));
-        }
+        nmit = -m * it;
+    };
+    // exponentials and sums of centre row r of a tile
+    auto row_sum = [&](const float (&z)[16], int r) {
+        t1 += ex2(z[r] - m);
+        if (TWO) t2 += ex2(fmaf(z[r], it, nmit));
+    };
+    // The empty asm pins the sums at this point: without it the compiler
+    // sinks the whole chain of exponentials below the next branch, out of the
+    // MFMA block they are meant to overlap.
+    auto tile_close = [&]() {
         kahan(S1, C1, t1);
-        if (two) kahan(S2, C2, t2);
-        if (t + 1 < ntiles) lstore(buf ^ 1);
-        __syncthreads();
+        if (TWO) kahan(S2, C2, t2);
+        asm volatile("" : "+v"(S1), "+v"(C1), "+v"(S2), "+v"(C2));
+        t1 = 0.f; t2 = 0.f;
+    };
+
+#pragma unroll
+    for (int i = 0; i < NIW; ++i) dma(0, 0, i);
+#pragma unroll
+    for (int i = 0; i < NIW; ++i) dma(1, 1, i);
+    __syncthreads();
+    float z[16];
+    {
+        f16v c;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[r] = 0.f;
+#pragma unroll
+        for (int mm = 0; mm < NX; ++mm) {
+            const int ks = mm / (TT + 1);
+            xmfma(mm, ldY(0, 0, ks), ldY(0, 1, ks), c);
+        }
+        tile_exp(0, c, z);
     }
+    // work steps spread evenly over the MFMA slots: 16 centre rows of tile t,
+    // then the DMA loads (slot m runs steps m W1 / NX .. (m + 1) W1 / NX - 1)
+    constexpr int W1 = 16 + NIW;
+    for (int t = 0; t + 1 < ntiles; ++t) {
+        const int s1 = (t + 1) % NS, s2 = (t + 2) % NS;
+        f16v c;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[r] = 0.f;
+        h8 ya0[KS], ya1[KS];
+#pragma unroll
+        for (int ks = 0; ks < 2 && ks < KS; ++ks) { ya0[ks] = ldY(s1, 0, ks); ya1[ks] = ldY(s1, 1, ks); }
+#pragma unroll
+        for (int mm = 0; mm < NX; ++mm) {
+            const int ks = mm / (TT + 1);
+            if (mm % (TT + 1) == 0 && ks + 2 < KS) {
+                ya0[ks + 2] = ldY(s1, 0, ks + 2);
+                ya1[ks + 2] = ldY(s1, 1, ks + 2);
+            }
+            xmfma(mm, ya0[ks], ya1[ks], c);
+#pragma unroll
+            for (int st = mm * W1 / NX; st < (mm + 1) * W1 / NX; ++st) {
+                if (st < 16) row_sum(z, st);
+                else dma(t + 2, s2, st - 16);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        tile_close();
+        tile_exp(s1, c, z);
+        __syncthreads();                           // also waits for this iteration's DMA
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) row_sum(z, r);
+    tile_close();
+
     // lanes j and j+32 hold the same point (different centre rows) and share m
     const float S1o = __shfl_xor(S1, 32), C1o = __shfl_xor(C1, 32);
     const float S2o = __shfl_xor(S2, 32), C2o = __shfl_xor(C2, 32);
@@ -197,7 +279,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
     float pbv = -INFINITY, ptv = -INFINITY;
     if (h == 0) {
         const double s1 = ((double)S1 - C1) + ((double)S1o - C1o);
-        const double s2 = two ? ((double)S2 - C2) + ((double)S2o - C2o) : s1;
+        const double s2 = TWO ? ((double)S2 - C2) + ((double)S2o - C2o) : s1;
         if (n >= N) {
         } else if (!(s1 > 0.0)) {          // zero column (mpicomm.c:382-390)
             lg = isDA ? -INFINITY : logK;
@@ -217,8 +299,9 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             pbv = (float)(-(double)it * m - L2 + kRLog2);
             ptv = (float)((double)it * ((double)thr - m) - L2 + kRLog2);
         }
-        pB[n] = pbv;
-        pThr[n] = ptv;
+        // per 32-point block: 32 exponent offsets pB, then 32 thresholds
+        pP[(n / BP) * (2 * BP) + j] = pbv;
+        pP[(n / BP) * (2 * BP) + BP + j] = ptv;
     }
     // This wave's 32 points are one pass-2 block: its largest mask threshold
     // tells pass 2 whether the block needs the underflow mask at all.
@@ -265,6 +348,15 @@ __global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__
     }
 }
 
+// Per-centre exponent term of pass 1: aK32[k] = -beta log2(e) |y_k|^2 / 2
+// (the same FP32 expression pass 2 uses), -inf for padding centres.
+__global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float beta,
+                          float *__restrict__ aK32) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float cb = -0.5f * beta * kLog2e;
+    if (k < Kp) aK32[k] = (k < K) ? cb * yy32[k] : -INFINITY;
+}
+
 // ---------------------------------------------------------------- pass 2
 // A workgroup owns 128 centres (32 per wave, both FP16 terms of Y held in
 // registers as the B operand of the cross term) and one chunk of points,
@@ -287,30 +379,34 @@ template <int KS, int TT, bool GM>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_e2_mfma(
     const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
-    const double *__restrict__ mscal, float beta, float it,
-    const float *__restrict__ pB, const float *__restrict__ pThr,
+    const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
     float *__restrict__ partRT, float *__restrict__ partG) {
     constexpr int ld = 16 * KS;
-    constexpr int LR = ld + PAD;
+    constexpr int LR = ld + PAD;                   // data row in LDS (halves), padding last
+    constexpr int RC = LR / 8;                     // 16-byte chunks per data row
     constexpr int DB = (ld + 31) / 32;
     constexpr int DR = 32 * DB;
-    constexpr int TR = BP + PAD;
-    constexpr int ROW8 = ld / 8;
-    constexpr int CHA = TT * BP * ROW8;
-    constexpr int CHB = TT * DR * (BP / 8);
-    constexpr int PA = (CHA + NTH - 1) / NTH;
-    constexpr int PB_ = (CHB + NTH - 1) / NTH;
-    constexpr int NS = 3;                          // LDS ring slots
-    __shared__ __attribute__((aligned(16))) _Float16 Ts[NS][TT][BP][LR];
-    __shared__ __attribute__((aligned(16))) _Float16 Tts[NS][TT][DR][TR];
-    __shared__ __attribute__((aligned(16))) float pBs[NS][BP];
-    __shared__ __attribute__((aligned(16))) float pTs[NS][BP];
+    constexpr int TR = BP + PAD;                   // transposed row in LDS (halves)
+    constexpr int TC = TR / 8;
+    constexpr int NIA = (TT * BP * RC + 63) / 64;  // 1 KB LDS-DMA loads: data rows,
+    constexpr int NIB = (TT * DR * TC + 63) / 64;  // transposed rows,
+    constexpr int NI = NIA + NIB + 1;              // pB and pThr
+    constexpr int NIW = (NI + 3) / 4;              // per wave
+    constexpr int SLOT = 4 * NIW * 512;            // halves per ring slot
+    constexpr int OFF_TT = NIA * 512, OFF_P = (NIA + NIB) * 512;
+    constexpr int NS = 3;                          // ring slots
+    constexpr int NX = (TT + 1) * KS;              // cross-term MFMAs per block
+    constexpr int NR = (TT + 1) * 2 * DB;          // R T MFMAs per block
+    static_assert(NS * SLOT * 2 <= 160 * 1024, "LDS ring");
+    __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
+
     const int L = blockIdx.x;
     const int xs = L & 7, sl = L >> 3;
     const int kg = sl % KG;
     const long cidx = (long)(sl / KG) * 8 + xs;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
     const int k = kg * CT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
@@ -331,144 +427,266 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const long nb = cidx * chunk;
     const long ne = (nb + chunk < Np) ? nb + chunk : Np;
     const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
+    const _Float16 *T16tc = T16t + cidx * (long)(TT * DR) * chunk;   // this chunk's T^T
 
-    h8 preA[PA], preB[PB_];
-    float pp = 0.f;
-    // global -> registers for block q (clamped into the chunk: a repeated
-    // block is loaded but never used)
-    auto gload = [&](int q) {
-        const long n0 = nb + (long)(q < nblocks ? q : nblocks - 1) * BP;
+    // Staging: block q goes global -> LDS slot s by LDS-DMA.  The slot is a
+    // sequence of 16-byte chunks [data rows | transposed rows | pB, pThr];
+    // DMA load I fills chunks 64 I .. 64 I + 63 in lane order, wave w issues
+    // loads w, w + 4, ...  Padding chunks (and the tail) take a copy of the
+    // load's first source chunk: never read.  Source offsets are fixed per
+    // lane; only the three block bases move.
+    unsigned voff[NIW];
 #pragma unroll
-        for (int p = 0; p < PA; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CHA) {
-                const int term = c / (BP * ROW8), rem = c % (BP * ROW8);
-                const int row = rem / ROW8, col = rem % ROW8;
-                preA[p] = *(const h8 *)(T16 + ((long)term * Np + n0 + row) * ld + col * 8);
-            }
+    for (int i = 0; i < NIW; ++i) {
+        const int I = wu + 4 * i;
+        unsigned o = 0;
+        if (I < NIA) {
+            const int g = I * 64 + l;
+            const int t = g / (BP * RC), rem = g % (BP * RC), row = rem / RC, col = rem % RC;
+            if (t < TT && col < ld / 8) o = (unsigned)((((long)t * Np + row) * ld + col * 8) * 2);
+        } else if (I < NIA + NIB) {
+            const int g = (I - NIA) * 64 + l;
+            const int t = g / (DR * TC), rem = g % (DR * TC), row = rem / TC, col = rem % TC;
+            if (t < TT && col < BP / 8) o = (unsigned)((((long)t * DR + row) * chunk + col * 8) * 2);
+        } else if (I == NIA + NIB) {
+            if (l < 2 * BP / 4) o = (unsigned)(l * 16);
         }
+        voff[i] = o;
+    }
+    // per DMA load of this wave: the source base at the chunk's first block
+    // and its step per block (both wave-uniform)
+    const char *dbase[NIW];
+    unsigned dstep[NIW];
 #pragma unroll
-        for (int p = 0; p < PB_; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CHB) {
-                const int term = c / (DR * (BP / 8)), rem = c % (DR * (BP / 8));
-                const int row = rem / (BP / 8), col = rem % (BP / 8);
-                preB[p] = *(const h8 *)(T16t + ((long)term * DR + row) * Np + n0 + col * 8);
-            }
-        }
-        if (tid < BP) pp = pB[n0 + tid];
-        else if (tid < 2 * BP) pp = pThr[n0 + tid - BP];
+    for (int i = 0; i < NIW; ++i) {
+        const int I = wu + 4 * i;
+        dbase[i] = (I < NIA) ? (const char *)(T16 + nb * ld)
+                 : (I < NIA + NIB) ? (const char *)T16tc
+                                   : (const char *)(pP + (nb / BP) * (2 * BP));
+        dstep[i] = (I < NIA) ? (unsigned)(BP * ld * 2) : (I < NIA + NIB) ? (unsigned)(BP * 2)
+                                                                       : (unsigned)(2 * BP * 4);
+    }
+    auto dma = [&](int q, int s, int i) {
+        const unsigned qc = (unsigned)(q < nblocks ? q : nblocks - 1);
+        const int I = wu + 4 * i;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
+            (__attribute__((address_space(3))) void *)&smem[s][I * 512], 16, 0, 0);
     };
-    auto lstore = [&](int s) {
-#pragma unroll
-        for (int p = 0; p < PA; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CHA) {
-                const int term = c / (BP * ROW8), rem = c % (BP * ROW8);
-                const int row = rem / ROW8, col = rem % ROW8;
-                *(h8 *)&Ts[s][term][row][col * 8] = preA[p];
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < PB_; ++p) {
-            const int c = tid + p * NTH;
-            if (c < CHB) {
-                const int term = c / (DR * (BP / 8)), rem = c % (DR * (BP / 8));
-                const int row = rem / (BP / 8), col = rem % (BP / 8);
-                *(h8 *)&Tts[s][term][row][col * 8] = preB[p];
-            }
-        }
-        if (tid < BP) pBs[s][tid] = pp;
-        else if (tid < 2 * BP) pTs[s][tid - BP] = pp;
+    // A operands: data rows (cross term) and transposed rows (R T)
+    auto ldT = [&](int s, int t, int ks) {
+        return *(const h8 *)&smem[s][(t * BP + j) * LR + 16 * ks + 8 * h];
     };
-    // cross term of the block in slot s -> exponents (base 2, R * 2^14 scale)
-    auto cross_exp = [&](int s, float (&e)[16]) {
-        f16v c;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) c[r] = 0.f;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const h8 a0 = *(const h8 *)&Ts[s][0][j][16 * ks + 8 * h];
-            c = mfma(a0, yb[0][ks], c);
-            c = mfma(a0, yb[1][ks], c);
-            if (TT == 2) {
-                const h8 a1 = *(const h8 *)&Ts[s][TT - 1][j][16 * ks + 8 * h];
-                c = mfma(a1, yb[0][ks], c);
-            }
-        }
-#pragma unroll
-        for (int m4 = 0; m4 < 4; ++m4) {
-            const f4v pb = *(const f4v *)&pBs[s][8 * m4 + 4 * h];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) e[4 * m4 + q] = fmaf(itc1, c[4 * m4 + q], aKi) + pb[q];
-        }
+    auto ldTt = [&](int s, int t, int g) {         // g = 2 db + sh
+        return *(const h8 *)&smem[s][OFF_TT + (t * DR + 32 * (g >> 1) + j) * TR + 16 * (g & 1) + 8 * h];
     };
-    // exp-before-pow underflow: elements below their point's threshold are 0
+    auto ldP = [&](int s, int which, int m4) {     // which: 0 pB, 1 pThr
+        return *(const f4v *)&smem[s][OFF_P + 2 * (which * BP + 8 * m4 + 4 * h)];
+    };
+    // cross-term MFMA number m (m = (TT + 1) ks + which)
+    auto xmfma = [&](int m, const h8 &a0, const h8 &a1, f16v &c) {
+        const int ks = m / (TT + 1), wq = m % (TT + 1);
+        if (wq == 0) c = mfma(a0, yb[0][ks], c);
+        else if (wq == 1) c = mfma(a0, yb[1][ks], c);
+        else c = mfma(a1, yb[0][ks], c);
+    };
+    // exponent (base 2, R * 2^14 scale) of element r from the cross term
+    auto expo = [&](const f16v &c, const f4v (&pb)[4], float (&e)[16], int r) {
+        e[r] = fmaf(itc1, c[r], aKi) + pb[r >> 2][r & 3];
+    };
+    // exp-before-pow underflow: elements below their point's threshold are 0.
+    // Called under a branch; the empty asm makes the thresholds opaque so the
+    // compares cannot be hoisted out of it.
     auto mask = [&](int s, float (&e)[16]) {
 #pragma unroll
         for (int m4 = 0; m4 < 4; ++m4) {
-            const f4v pt = *(const f4v *)&pTs[s][8 * m4 + 4 * h];
+            f4v pt = ldP(s, 1, m4);
+            asm volatile("" : "+v"(pt));
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 e[4 * m4 + q] = (e[4 * m4 + q] < pt[q]) ? -INFINITY : e[4 * m4 + q];
         }
     };
-    // R = 2^e as FP16 hi/lo pairs in the B-operand order of the R T product
-    auto resp = [&](const float (&e)[16], h8 (&rhi)[2], h8 (&rlo)[2]) {
-        float gs = 0.f;
-#pragma unroll
-        for (int r = 0; r < 16; r += 2) {
-            const float R0 = ex2(e[r]), R1 = ex2(e[r + 1]);
-            if (!GM) gs += R0 + R1;
-            const h2 hp = {(_Float16)R0, (_Float16)R1};
-            const h2 lp = {(_Float16)(R0 - (float)hp[0]), (_Float16)(R1 - (float)hp[1])};
-            rhi[r >> 3][r & 7] = hp[0]; rhi[r >> 3][(r & 7) + 1] = hp[1];
-            rlo[r >> 3][r & 7] = lp[0]; rlo[r >> 3][(r & 7) + 1] = lp[1];
+    // R = 2^e as FP16 hi/lo pairs in the B-operand order of the R T product,
+    // in 16 steps: step 2p takes the exponentials of elements 2p, 2p+1, step
+    // 2p+1 packs hi = RNE(R) and lo = RNE(R - hi) (v_fma_mix reads one FP16
+    // half of the packed hi and writes one half of lo).
+    float Rv[16];
+    unsigned wh[8], wl[8];
+    float gs = 0.f;
+    auto resp_step = [&](const float (&e)[16], int st) {
+        const int p = st >> 1;
+        if ((st & 1) == 0) {
+            Rv[2 * p] = ex2(e[2 * p]);
+            Rv[2 * p + 1] = ex2(e[2 * p + 1]);
+            if (!GM) gs += Rv[2 * p] + Rv[2 * p + 1];
+        } else {
+            const h2 hp = {(_Float16)Rv[2 * p], (_Float16)Rv[2 * p + 1]};
+            const unsigned hb = __builtin_bit_cast(unsigned, hp);
+            unsigned lb = 0u;
+            asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "+v"(lb) : "v"(hb), "v"(Rv[2 * p]));
+            asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+                : "+v"(lb) : "v"(hb), "v"(Rv[2 * p + 1]));
+            wh[p] = hb;
+            wl[p] = lb;
         }
-        if (!GM) kahan(G, Gc, gs);
     };
-    auto rt = [&](int s, const h8 (&rhi)[2], const h8 (&rlo)[2]) {
+    auto resp_ops = [&](h8 (&rhi)[2], h8 (&rlo)[2]) {
+        // The compiler's hazard recognizer does not see the v_fma_mix writes
+        // inside the asm, so it would not insert the two wait states an MFMA
+        // needs before reading a VGPR that a VALU instruction has just
+        // written: fence the scheduler and wait here, once per block.
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_nop 1");
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int db = 0; db < DB; ++db) {
+        for (int sh = 0; sh < 2; ++sh) {
+            const u4 a = {wh[4 * sh], wh[4 * sh + 1], wh[4 * sh + 2], wh[4 * sh + 3]};
+            const u4 b = {wl[4 * sh], wl[4 * sh + 1], wl[4 * sh + 2], wl[4 * sh + 3]};
+            rhi[sh] = __builtin_bit_cast(h8, a);
+            rlo[sh] = __builtin_bit_cast(h8, b);
+        }
+        if (!GM) { kahan(G, Gc, gs); gs = 0.f; }
+    };
+    // R T MFMAs: accumulator d-block db, 16-point half sh, term wq (hi R,
+    // lo R, second data term).  Issue order: by sh, then by pairs of d-blocks,
+    // alternating the two accumulators of a pair so that no MFMA waits for
+    // the one just before it.
+    struct RtOp { int db, sh, wq, pairFirst; };
+    auto rt_op = [&](int i) {
+        constexpr int GP = TT + 1;
+        const int sh = i / (DB * GP), r = i % (DB * GP);
+        const int pair = r / (2 * GP), in = r % (2 * GP);
+        const int npd = (2 * pair + 1 < DB) ? 2 : 1;
+        return RtOp{2 * pair + in % npd, sh, in / npd, in == 0};
+    };
+    auto rmfma = [&](const RtOp &o, const h8 &u0, const h8 &u1, const h8 (&rhi)[2],
+                     const h8 (&rlo)[2]) {
+        if (o.wq == 0) acc[o.db] = mfma(u0, rhi[o.sh], acc[o.db]);
+        else if (o.wq == 1) acc[o.db] = mfma(u0, rlo[o.sh], acc[o.db]);
+        else acc[o.db] = mfma(u1, rhi[o.sh], acc[o.db]);
+    };
+    // whole-block versions (prologue and epilogue)
+    auto cross_exp = [&](int s, float (&e)[16]) {
+        f16v c;
 #pragma unroll
-            for (int sh = 0; sh < 2; ++sh) {
-                const h8 a0 = *(const h8 *)&Tts[s][0][32 * db + j][16 * sh + 8 * h];
-                acc[db] = mfma(a0, rhi[sh], acc[db]);
-                acc[db] = mfma(a0, rlo[sh], acc[db]);
-                if (TT == 2) {
-                    const h8 a1 = *(const h8 *)&Tts[s][TT - 1][32 * db + j][16 * sh + 8 * h];
-                    acc[db] = mfma(a1, rhi[sh], acc[db]);
-                }
-            }
+        for (int r = 0; r < 16; ++r) c[r] = 0.f;
+#pragma unroll
+        for (int m = 0; m < NX; ++m) {
+            const int ks = m / (TT + 1);
+            xmfma(m, ldT(s, 0, ks), ldT(s, TT - 1, ks), c);
+        }
+        f4v pb[4];
+#pragma unroll
+        for (int m4 = 0; m4 < 4; ++m4) pb[m4] = ldP(s, 0, m4);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) expo(c, pb, e, r);
+    };
+    auto resp_rt = [&](int s, const float (&e)[16]) {
+#pragma unroll
+        for (int st = 0; st < 16; ++st) resp_step(e, st);
+        h8 rhi[2], rlo[2];
+        resp_ops(rhi, rlo);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const RtOp o = rt_op(i);
+            const int g = 2 * o.db + o.sh;
+            rmfma(o, ldTt(s, 0, g), ldTt(s, TT - 1, g), rhi, rlo);
         }
     };
     const long blk0 = nb / BP;                     // index of the chunk's first block
 
     if (nblocks > 0) {
-        gload(0); lstore(0);
-        gload(1); lstore(1);
-        gload(2);
+#pragma unroll
+        for (int i = 0; i < NIW; ++i) dma(0, 0, i);
+#pragma unroll
+        for (int i = 0; i < NIW; ++i) dma(1, 1, i);
         __syncthreads();
         float e[16];
         cross_exp(0, e);
         if (blkThr[blk0] > -kMaskMargin) mask(0, e);
+        // Steady state, one block per iteration, in a fixed issue order
+        // (sched_barrier keeps the compiler from regrouping it):
+        //   phase 1: cross-term MFMA m of block b+1, then step m of: block b's
+        //            exponentials and packs (16 steps) and the DMA loads of
+        //            block b+2;
+        //   phase 2: R T MFMA i of block b, then step i of block b+1's
+        //            exponents (pB loads, 16 elements).
+        // The A operands are read from LDS two MFMAs ahead.
+        // work steps are spread evenly over the MFMA slots: slot m of a phase
+        // with W steps runs steps m W / n .. (m + 1) W / n - 1
+        constexpr int W1 = 16 + NIW;               // phase 1: R steps, DMA loads
+        constexpr int W2 = 4 + 16;                 // phase 2: pB loads, exponents
         for (int b = 0; b + 1 < nblocks; ++b) {
             const int s0 = b % NS, s1 = (b + 1) % NS, s2 = (b + 2) % NS;
+            f16v c;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) c[r] = 0.f;
+            h8 xa0[KS], xa1[KS];
+#pragma unroll
+            for (int ks = 0; ks < 2 && ks < KS; ++ks) {
+                xa0[ks] = ldT(s1, 0, ks);
+                if (TT == 2) xa1[ks] = ldT(s1, 1, ks);
+            }
+#pragma unroll
+            for (int m = 0; m < NX; ++m) {
+                const int ks = m / (TT + 1);
+                if (m % (TT + 1) == 0 && ks + 2 < KS) {
+                    xa0[ks + 2] = ldT(s1, 0, ks + 2);
+                    if (TT == 2) xa1[ks + 2] = ldT(s1, 1, ks + 2);
+                }
+                xmfma(m, xa0[ks], TT == 2 ? xa1[ks] : xa0[ks], c);
+#pragma unroll
+                for (int st = m * W1 / NX; st < (m + 1) * W1 / NX; ++st) {
+                    if (st < 16) resp_step(e, st);
+                    else dma(b + 2, s2, st - 16);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
             h8 rhi[2], rlo[2];
-            resp(e, rhi, rlo);                     // block b: R from its exponents
+            resp_ops(rhi, rlo);
+            // A operands, indexed g = 2 db + sh; the first pair's were read
+            // during phase 1, each pair's first MFMA reads the next pair's
+            h8 ru0[2 * DB], ru1[2 * DB];
+            auto ld_pair = [&](int i0) {           // operands of the pair starting at MFMA i0
+                const RtOp o = rt_op(i0);
+                const int npd = (o.db + 1 < DB && rt_op(i0 + 1).db != o.db) ? 2 : 1;
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    if (x < npd) {
+                        const int g = 2 * (o.db + x) + o.sh;
+                        ru0[g] = ldTt(s0, 0, g);
+                        if (TT == 2) ru1[g] = ldTt(s0, 1, g);
+                    }
+            };
+            ld_pair(0);
+            f4v pb[4];
             float en[16];
-            cross_exp(s1, en);                     // block b+1: cross term
-            rt(s0, rhi, rlo);                      // block b: R T
-            lstore(s2);                            // block b+2 into the free slot
-            gload(b + 3);
+#pragma unroll
+            for (int i = 0; i < NR; ++i) {
+                const RtOp o = rt_op(i);
+                if (o.pairFirst) {
+                    // the next pair starts after this one's 2 (TT + 1) MFMAs (or fewer)
+                    int nx = i + 1;
+                    while (nx < NR && !rt_op(nx).pairFirst) ++nx;
+                    if (nx < NR) ld_pair(nx);
+                }
+                const int g = 2 * o.db + o.sh;
+                rmfma(o, ru0[g], TT == 2 ? ru1[g] : ru0[g], rhi, rlo);
+#pragma unroll
+                for (int st = i * W2 / NR; st < (i + 1) * W2 / NR; ++st) {
+                    if (st < 4) pb[st] = ldP(s1, 0, st);
+                    else expo(c, pb, en, st - 4);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // rarely taken; a real branch after every MFMA of the iteration
             if (blkThr[blk0 + b + 1] > -kMaskMargin) mask(s1, en);
 #pragma unroll
             for (int r = 0; r < 16; ++r) e[r] = en[r];
-            __syncthreads();
+            __syncthreads();                       // also waits for this iteration's DMA
         }
-        h8 rhi[2], rlo[2];
-        resp(e, rhi, rlo);
-        rt((nblocks - 1) % NS, rhi, rlo);
+        resp_rt((nblocks - 1) % NS, e);
     }
     // the accumulators are read after the loop's exit branch: keep the
     // scheduler from moving those reads up and give the MFMA pipe time to drain
@@ -614,13 +832,15 @@ __global__ void k_make_t16(const double *__restrict__ T64, int ldt, long N, long
     tt32[n] = (float)ss;
 }
 
-// T16t[t][d][pos(n)] = T16[t][n][d] for d < DR (rows >= ld16 zero), with the
-// points of each 16-point group stored in the order 0-3, 8-11, 4-7, 12-15
-// (pos swaps the two middle quads) so that a lane's eight R-operand points
-// are one 16-byte run.  With ones_row >= 0, row ones_row of term 0 is 1 for
-// real points (n < N) and 0 for padding.  64 x 64 tiles.
+// Transposed data, chunk-major: for point n in chunk c = n / CL (offset
+// o = n % CL), T16t[((c TT + t) DR + d) CL + pos(o)] = T16[t][n][d] for d < DR
+// (rows >= ld16 zero), with the points of each 16-point group stored in the
+// order 0-3, 8-11, 4-7, 12-15 (pos swaps the two middle quads) so that a
+// lane's eight R-operand points are one 16-byte run.  With ones_row >= 0,
+// row ones_row of term 0 is 1 for real points (n < N) and 0 for padding.
+// 64 x 64 tiles; CL is a multiple of 64.
 __global__ void k_transpose16(const _Float16 *__restrict__ T16, long N, long Np, int ld16, int DR,
-                              int ones_row, _Float16 *__restrict__ T16t) {
+                              int ones_row, long CL, int TT, _Float16 *__restrict__ T16t) {
     __shared__ _Float16 tile[64][65];
     const long n0 = (long)blockIdx.x * 64;
     const int d0 = blockIdx.y * 64;
@@ -634,13 +854,14 @@ __global__ void k_transpose16(const _Float16 *__restrict__ T16, long N, long Np,
         tile[r][c] = v;
     }
     __syncthreads();
+    const long ch = n0 / CL, o0 = n0 % CL;
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
         const int r = e / 64, c = e % 64;           // r: dim, c: stored position
         const int q = (c >> 2) & 3;
         const int src = (c & ~15) | ((((q & 1) << 1) | (q >> 1)) << 2) | (c & 3);
         const int d = d0 + r;
-        const long n = n0 + c;
-        if (d < DR && n < Np) T16t[((long)t * DR + d) * Np + n] = tile[src][r];
+        if (d < DR && n0 + c < Np)
+            T16t[((ch * TT + t) * DR + d) * CL + o0 + c] = tile[src][r];
     }
 }
 
@@ -683,8 +904,10 @@ struct MixedBufs {
     double *mscal = nullptr;
     double *partLog = nullptr;    // 3 per pass-1 workgroup
     float *blkThr = nullptr;      // per 32-point block: largest mask threshold
+    float *aK32 = nullptr;        // Kp: pass-1 per-centre exponent term
     double *amaxPart = nullptr;   // 2 x kAmaxWG
     long Np = 0;
+    long CL = 0;                  // points per pass-2 chunk (multiple of 64)
     int Kp = 0, KS = 0, TT = 1, NC = 1;
 };
 
@@ -693,25 +916,32 @@ MixedBufs *mb(dg_ctx *c) { return reinterpret_cast<MixedBufs *>(c->mix); }
 template <int KS, int TT>
 void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int isDA,
                   double logK) {
-    hipLaunchKernelGGL((k_e1_mfma<KS, TT>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0, c->stream,
-                       (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16, c->yy32, c->K,
-                       b->Kp, b->mscal, beta, it, two, isDA, logK, c->pm32, c->pl32, b->blkThr,
-                       c->pm, c->logg, c->logh, b->partLog);
+    hipLaunchKernelGGL(k_make_ak, dim3((unsigned)((b->Kp + 255) / 256)), dim3(256), 0, c->stream,
+                       c->yy32, c->K, b->Kp, beta, b->aK32);
+    if (two)
+        hipLaunchKernelGGL((k_e1_mfma<KS, TT, true>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
+                           c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
+                           b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
+                           c->pm, c->logg, c->logh, b->partLog);
+    else
+        hipLaunchKernelGGL((k_e1_mfma<KS, TT, false>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
+                           c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
+                           b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
+                           c->pm, c->logg, c->logh, b->partLog);
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
-    const long blocks = b->Np / BP;
-    const long chunk = ((blocks + b->NC - 1) / b->NC) * BP;
+    const long chunk = b->CL;
     const int KG = b->Kp / CT;
     // G comes out of the R T product when the padded d range has a spare row
     constexpr int DR = 32 * ((16 * KS + 31) / 32);
     if (DR > c->D)
         hipLaunchKernelGGL((k_e2_mfma<KS, TT, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
-                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pm32, c->pl32, b->blkThr,
+                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
                            chunk, KG, c->partRT32, c->partG32);
     else
         hipLaunchKernelGGL((k_e2_mfma<KS, TT, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
-                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pm32, c->pl32, b->blkThr,
+                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
                            chunk, KG, c->partRT32, c->partG32);
 }
 
@@ -762,6 +992,7 @@ void dg_mixed_free(dg_ctx *c) {
         if (b->mscal) (void)hipFree(b->mscal);
         if (b->partLog) (void)hipFree(b->partLog);
         if (b->blkThr) (void)hipFree(b->blkThr);
+        if (b->aK32) (void)hipFree(b->aK32);
         if (b->amaxPart) (void)hipFree(b->amaxPart);
         delete b;
     }
@@ -769,12 +1000,11 @@ void dg_mixed_free(dg_ctx *c) {
     if (c->T16) (void)hipFree(c->T16);
     if (c->tt32) (void)hipFree(c->tt32);
     if (c->yy32) (void)hipFree(c->yy32);
-    if (c->pm32) (void)hipFree(c->pm32);
-    if (c->pl32) (void)hipFree(c->pl32);
+    if (c->pP32) (void)hipFree(c->pP32);
     if (c->partRT32) (void)hipFree(c->partRT32);
     if (c->partG32) (void)hipFree(c->partG32);
-    c->T16 = nullptr; c->tt32 = nullptr; c->yy32 = nullptr; c->pm32 = nullptr;
-    c->pl32 = nullptr; c->partRT32 = nullptr; c->partG32 = nullptr;
+    c->T16 = nullptr; c->tt32 = nullptr; c->yy32 = nullptr; c->pP32 = nullptr;
+    c->partRT32 = nullptr; c->partG32 = nullptr;
     c->mixed_ready = 0;
 }
 
@@ -800,6 +1030,8 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     if (nc < 1) nc = 1;
     nc = ((nc + 7) / 8) * 8;          // the pass-2 grid maps 8 chunks per XCD round
     b->NC = (int)nc;
+    // chunk length: a multiple of 64 points (transpose tiles never straddle chunks)
+    b->CL = ((b->Np + nc * 64 - 1) / (nc * 64)) * 64;
 
     int rc;
     if ((rc = dalloc(c, &b->amaxPart, (size_t)2 * kAmaxWG))) return rc;
@@ -820,18 +1052,22 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         return dg_fail(c, DG_EINVAL,
                        "mixed path: data not exact in FP16 is limited to D <= 128 "
                        "(use DG_FP64, or FP16-exact data such as binary fingerprints)");
+    // pass 2 addresses the data rows with 32-bit byte offsets
+    if ((double)b->TT * (double)b->Np * ld16 * 2.0 >= 4294967296.0)
+        return dg_fail(c, DG_EINVAL, "mixed path: N x D too large for one context (split N)");
     double tscale = 1.0;
     if (!c->t_exact && amax > 0.0) tscale = ldexp(1.0, 13 - ilogb(amax));
     c->t_scale = (float)tscale;
 
     if ((rc = dalloc(c, (_Float16 **)&c->T16, (size_t)b->TT * b->Np * ld16)) ||
-        (rc = dalloc(c, &b->T16t, (size_t)b->TT * DR * b->Np)) ||
+        (rc = dalloc(c, &b->T16t, (size_t)b->NC * b->TT * DR * b->CL)) ||
         (rc = dalloc(c, &c->tt32, (size_t)b->Np)) ||
         (rc = dalloc(c, &b->Y16, (size_t)2 * b->Kp * ld16)) ||
         (rc = dalloc(c, &c->yy32, (size_t)b->Kp)) || (rc = dalloc(c, &b->mscal, 4)) ||
         (rc = dalloc(c, &b->partLog, (size_t)3 * (b->Np / PT))) ||
         (rc = dalloc(c, &b->blkThr, (size_t)(b->Np / BP))) ||
-        (rc = dalloc(c, &c->pm32, (size_t)b->Np)) || (rc = dalloc(c, &c->pl32, (size_t)b->Np)) ||
+        (rc = dalloc(c, &b->aK32, (size_t)b->Kp)) ||
+        (rc = dalloc(c, &c->pP32, (size_t)2 * b->Np)) ||
         (rc = dalloc(c, &c->partRT32, (size_t)b->NC * c->K * c->D)) ||
         (rc = dalloc(c, &c->partG32, (size_t)b->NC * c->K)))
         return rc;
@@ -845,7 +1081,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
                        dim3((unsigned)((b->Np + 63) / 64), (unsigned)((DR + 63) / 64),
                             (unsigned)b->TT),
                        dim3(256), 0, c->stream, (const _Float16 *)c->T16, c->N, b->Np, ld16, DR,
-                       DR > c->D ? c->D : -1, b->T16t);
+                       DR > c->D ? c->D : -1, b->CL, b->TT, b->T16t);
     DG_HIP(c, hipGetLastError());
     DG_HIP(c, hipStreamSynchronize(c->stream));
     c->mixed_ready = 1;

@@ -1,0 +1,97 @@
+"""Checks on the gfx950 assembly of the mixed-precision kernels (built here).
+
+The FP16 lo-term split of pass 2 is inline asm (v_fma_mixlo/mixhi_f16), and
+the compiler's hazard recognizer does not count inline asm as a VALU write: it
+would insert no wait states before an MFMA that reads such a VGPR, although
+the hardware needs two.  The kernel fences the split off with a scheduling
+barrier and an s_nop; this test compiles the source to assembly and checks, for
+every instantiation of the pass-2 kernel, that no MFMA reads a VGPR written by
+a v_fma_mix instruction fewer than three instructions earlier in the same
+straight-line stretch.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "da-gtm_amd", "csrc", "dg_mixed.hip")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+def _regs(tok):
+    m = re.match(r"v\[(\d+):(\d+)\]", tok)
+    if m:
+        return [f"v{r}" for r in range(int(m.group(1)), int(m.group(2)) + 1)]
+    m = re.match(r"(v\d+)\b", tok)
+    return [m.group(1)] if m else []
+
+
+def _min_mix_to_mfma(lines):
+    """Smallest instruction distance from a v_fma_mix write to an MFMA A/B read."""
+    last, worst = {}, None
+    for k, x in enumerate(lines):
+        if x.startswith(("s_branch", "s_cbranch")) or x.endswith(":"):
+            last = {}
+            continue
+        op, _, rest = x.partition(" ")
+        toks = [t.strip() for t in rest.split(",")]
+        if op.startswith("v_fma_mix"):
+            for r in _regs(toks[0]):
+                last[r] = k
+        if "mfma" in op and len(toks) >= 3:
+            for t in toks[1:3]:
+                for r in _regs(t):
+                    if r in last and (worst is None or k - last[r] < worst):
+                        worst = k - last[r]
+    return worst
+
+
+@pytest.fixture(scope="module")
+def asm(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("isa") / "dg_mixed.s"
+    cmd = [HIPCC, "-O3", "-std=c++17", "--offload-arch=gfx950", "--cuda-device-only", "-S",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.dirname(SRC), SRC, "-o", str(out)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=600)
+    return out.read_text().split("\n")
+
+
+def _kernels(lines, stem):
+    out, cur = {}, None
+    for l in lines:
+        m = re.match(r"^(_Z\w*" + stem + r"\w*):", l)
+        if m:
+            cur = m.group(1)
+            out[cur] = []
+            continue
+        if cur is None:
+            continue
+        x = l.strip()
+        if x.startswith(".Lfunc_end"):
+            cur = None
+            continue
+        if x and not x.startswith(";") and (not x.startswith(".") or re.match(r"^\.LBB\d+_\d+:", x)):
+            out[cur].append(x)
+    return out
+
+
+def test_mix_split_is_fenced_from_mfma(asm):
+    ks = _kernels(asm, "k_e2_mfma")
+    assert len(ks) >= 16, "pass-2 instantiations not found"
+    for name, body in ks.items():
+        d = _min_mix_to_mfma(body)
+        assert d is not None, name            # every pass-2 kernel uses the split
+        assert d >= 3, (name, d)
+
+
+def test_checker_sees_an_unfenced_read():
+    """The checker itself: a v_fma_mix result read by the next MFMA is caught."""
+    body = ["v_cvt_pk_f16_f32 v4, v1, v2",
+            "v_fma_mixlo_f16 v3, v4, -1.0, v1 op_sel_hi:[1,0,0]",
+            "v_mfma_f32_32x32x16_f16 a[0:15], v[20:23], v[0:3], a[0:15]"]
+    assert _min_mix_to_mfma(body) == 1
+    assert _min_mix_to_mfma(body[:2] + ["s_nop 1", "s_branch .LBB0_1", ".LBB0_1:"] + body[2:]) is None
