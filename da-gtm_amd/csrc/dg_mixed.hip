@@ -203,10 +203,13 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         }
         nmit = -m * it;
     };
-    // exponentials and sums of centre row r of a tile
-    auto row_sum = [&](const float (&z)[16], int r) {
-        t1 += ex2(z[r] - m);
-        if (TWO) t2 += ex2(fmaf(z[r], it, nmit));
+    // exponentials and sums of a tile, in steps: step 2r adds centre row r to
+    // the first sum, step 2r+1 to the second (nothing when !TWO)
+    constexpr int NRW = 32;
+    auto row_sum = [&](const float (&z)[16], int st) {
+        const int r = st >> 1;
+        if ((st & 1) == 0) t1 += ex2(z[r] - m);
+        else if (TWO) t2 += ex2(fmaf(z[r], it, nmit));
     };
     // The empty asm pins the sums at this point: without it the compiler
     // sinks the whole chain of exponentials below the next branch, out of the
@@ -235,9 +238,9 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         }
         tile_exp(0, c, z);
     }
-    // work steps spread evenly over the MFMA slots: 16 centre rows of tile t,
+    // work steps spread evenly over the MFMA slots: the row sums of tile t,
     // then the DMA loads (slot m runs steps m W1 / NX .. (m + 1) W1 / NX - 1)
-    constexpr int W1 = 16 + NIW;
+    constexpr int W1 = NRW + NIW;
     for (int t = 0; t + 1 < ntiles; ++t) {
         const int s1 = (t + 1) % NS, s2 = (t + 2) % NS;
         f16v c;
@@ -256,8 +259,8 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
             xmfma(mm, ya0[ks], ya1[ks], c);
 #pragma unroll
             for (int st = mm * W1 / NX; st < (mm + 1) * W1 / NX; ++st) {
-                if (st < 16) row_sum(z, st);
-                else dma(t + 2, s2, st - 16);
+                if (st < NRW) row_sum(z, st);
+                else dma(t + 2, s2, st - NRW);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         __syncthreads();                           // also waits for this iteration's DMA
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) row_sum(z, r);
+    for (int st = 0; st < NRW; ++st) row_sum(z, st);
     tile_close();
 
     // lanes j and j+32 hold the same point (different centre rows) and share m
@@ -508,18 +511,18 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         }
     };
     // R = 2^e as FP16 hi/lo pairs in the B-operand order of the R T product,
-    // in 16 steps: step 2p takes the exponentials of elements 2p, 2p+1, step
-    // 2p+1 packs hi = RNE(R) and lo = RNE(R - hi) (v_fma_mix reads one FP16
-    // half of the packed hi and writes one half of lo).
+    // in 24 steps, three per pair p: the exponentials of elements 2p and
+    // 2p+1, then the packs hi = RNE(R) and lo = RNE(R - hi) (v_fma_mix reads
+    // one FP16 half of the packed hi and writes one half of lo).
+    constexpr int NRS = 24;
     float Rv[16];
     unsigned wh[8], wl[8];
     float gs = 0.f;
     auto resp_step = [&](const float (&e)[16], int st) {
-        const int p = st >> 1;
-        if ((st & 1) == 0) {
-            Rv[2 * p] = ex2(e[2 * p]);
-            Rv[2 * p + 1] = ex2(e[2 * p + 1]);
-            if (!GM) gs += Rv[2 * p] + Rv[2 * p + 1];
+        const int p = st / 3, q = st % 3;
+        if (q < 2) {
+            Rv[2 * p + q] = ex2(e[2 * p + q]);
+            if (!GM) gs += Rv[2 * p + q];
         } else {
             const h2 hp = {(_Float16)Rv[2 * p], (_Float16)Rv[2 * p + 1]};
             const unsigned hb = __builtin_bit_cast(unsigned, hp);
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     };
     auto resp_rt = [&](int s, const float (&e)[16]) {
 #pragma unroll
-        for (int st = 0; st < 16; ++st) resp_step(e, st);
+        for (int st = 0; st < NRS; ++st) resp_step(e, st);
         h8 rhi[2], rlo[2];
         resp_ops(rhi, rlo);
 #pragma unroll
@@ -615,7 +618,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         // The A operands are read from LDS two MFMAs ahead.
         // work steps are spread evenly over the MFMA slots: slot m of a phase
         // with W steps runs steps m W / n .. (m + 1) W / n - 1
-        constexpr int W1 = 16 + NIW;               // phase 1: R steps, DMA loads
+        constexpr int W1 = NRS + NIW;              // phase 1: R steps, DMA loads
         constexpr int W2 = 4 + 16;                 // phase 2: pB loads, exponents
         for (int b = 0; b + 1 < nblocks; ++b) {
             const int s0 = b % NS, s1 = (b + 1) % NS, s2 = (b + 2) % NS;
@@ -638,8 +641,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 xmfma(m, xa0[ks], TT == 2 ? xa1[ks] : xa0[ks], c);
 #pragma unroll
                 for (int st = m * W1 / NX; st < (m + 1) * W1 / NX; ++st) {
-                    if (st < 16) resp_step(e, st);
-                    else dma(b + 2, s2, st - 16);
+                    if (st < NRS) resp_step(e, st);
+                    else dma(b + 2, s2, st - NRS);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1021,7 +1024,10 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     c->ldh = ld16;
     b->Np = ((c->N + PT - 1) / PT) * PT;
     b->Kp = ((c->K + CT - 1) / CT) * CT;
-    // chunks of pass 2: ~8K points each, enough workgroups to fill the chip
+    // chunks of pass 2: ~8K points each (the FP32 accumulation of R T over a
+    // chunk carries a small systematic bias that grows with its length; at 16K
+    // points the annealing dynamics amplified it to 3e-6 in llh within four
+    // iterations), enough workgroups to fill the chip
     const long blocks = b->Np / BP;
     long nc = (c->N + 8191) / 8192;
     const long want = (2048 + (b->Kp / CT) - 1) / (b->Kp / CT);

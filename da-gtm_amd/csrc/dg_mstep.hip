@@ -148,57 +148,94 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double *__restrict__ 
     }
 }
 
-// Split-K partials of A = FI^T diag(G) FI and B = FI^T RT: chunk c covers
-// centres [c*kc, min(K, (c+1)*kc)); entry e < M1*M1 is A[e / M1][e % M1],
-// the rest B[(e - M1*M1) / D][(e - M1*M1) % D].  part is NCH x E.
-__global__ void k_partAB(const double *__restrict__ FI, const double *__restrict__ G,
-                         const double *__restrict__ RT, int K, int M1, int D, int kc,
-                         double *__restrict__ part) {
-    const long EA = (long)M1 * M1, E = EA + (long)M1 * D;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (e >= E) return;
-    const int k0 = c * kc;
-    const int k1 = (k0 + kc < K) ? k0 + kc : K;
-    double s = 0.0;
-    if (e < EA) {
-        const int i = (int)(e / M1), j = (int)(e % M1);
-        for (int k = k0; k < k1; ++k) {
-            const double g = FI[(long)k * M1 + j] * G[k];
-            s = fma(FI[(long)k * M1 + i], g, s);
+// Tiled FP64 products of the M-step: 64 x 64 output tiles, 256 threads with
+// 4 x 4 results each (rows ty + 16 a, columns tx + 16 b: conflict-free LDS
+// reads), the inner dimension in steps of 16 staged through LDS.
+constexpr int GT = 64, GK = 16;
+
+// Split-K partials of [A | B] = FI^T [diag(G) FI | RT], an M1 x (M1 + D)
+// product over the centres: workgroup (bx, by, c) computes one output tile
+// over centres [c kc, min(K, (c + 1) kc)); part is nch x M1 x (M1 + D).
+__global__ __launch_bounds__(256) void k_partAB(const double *__restrict__ FI,
+                                                const double *__restrict__ G,
+                                                const double *__restrict__ RT, int K, int M1,
+                                                int D, int kc, double *__restrict__ part) {
+    __shared__ double As[GK][GT], Bs[GK][GT];
+    const int NCOL = M1 + D;
+    const int i0 = blockIdx.x * GT, j0 = blockIdx.y * GT, c = blockIdx.z;
+    const int k0 = c * kc, k1 = (k0 + kc < K) ? k0 + kc : K;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int kb = k0; kb < k1; kb += GK) {
+        for (int e = tid; e < GK * GT; e += 256) {
+            const int kk = e / GT, col = e % GT, k = kb + kk;
+            double av = 0.0, bv = 0.0;
+            if (k < k1) {
+                const int i = i0 + col, jj = j0 + col;
+                if (i < M1) av = FI[(long)k * M1 + i];
+                if (jj < M1) bv = G[k] * FI[(long)k * M1 + jj];
+                else if (jj < NCOL) bv = RT[(long)k * D + (jj - M1)];
+            }
+            As[kk][col] = av;
+            Bs[kk][col] = bv;
         }
-    } else {
-        const long e2 = e - EA;
-        const int i = (int)(e2 / D), d = (int)(e2 % D);
-        for (int k = k0; k < k1; ++k) s = fma(FI[(long)k * M1 + i], RT[(long)k * D + d], s);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
     }
-    part[(long)c * E + e] = s;
+    const long E = (long)M1 * NCOL;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            if (i < M1 && j < NCOL) part[(long)c * E + (long)i * NCOL + j] = acc[a][b];
+        }
 }
 
 // A0 / B from the chunk partials, summed in chunk order.
 __global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
                         double *__restrict__ A0, double *__restrict__ B) {
-    const long EA = (long)M1 * M1, E = EA + (long)M1 * D;
+    const int NCOL = M1 + D;
+    const long E = (long)M1 * NCOL;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     double s = 0.0;
     for (int c = 0; c < nch; ++c) s += part[(long)c * E + e];
-    if (e < EA) A0[e] = s;
-    else B[e - EA] = s;
+    const int i = (int)(e / NCOL), j = (int)(e % NCOL);
+    if (j < M1) A0[(long)i * M1 + j] = s;
+    else B[(long)i * D + (j - M1)] = s;
 }
 
 constexpr int kLdsN = 126;   // largest M+1 whose factor fits in LDS
 
 // Cholesky of A0 + (ridge + jitter) I with the matrix resident in LDS (one
-// workgroup, n <= 126); same pivots, retries and update order as
-// k_chol_solve.  Writes the factor's lower triangle to L.
+// workgroup, n <= 126); same pivots and retries as k_chol_solve.  One barrier
+// per column: step j updates the trailing lower triangle with the column
+// still unscaled, A[i][k] -= A[i][j] A[k][j] / p_j, and scales column j - 1,
+// which no step reads any more.  Writes the factor's lower triangle to L.
 __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
                                                    double *__restrict__ L, int n, double ridge,
                                                    double *__restrict__ scal) {
     __shared__ double As[kLdsN * kLdsN];
-    __shared__ int fail;
     __shared__ double tr;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int tx = tid & 31, ty = tid >> 5;          // 32 x 32 threads over (row, column)
     if (tid == 0) {
         double s = 0.0;
         for (int i = 0; i < n; ++i) s += A0[(long)i * n + i];
@@ -214,32 +251,33 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
             const int i = e / n, j = e % n;
             As[e] = A0[e] + ((i == j) ? ridge + jit : 0.0);
         }
-        if (tid == 0) fail = 0;
         __syncthreads();
+        bool failed = false;
+        double pprev = 0.0;
         for (int j = 0; j < n; ++j) {
-            if (tid == 0) {
-                const double p = As[j * n + j];
-                if (!(p > 0.0)) fail = 1;
-                else As[j * n + j] = sqrt(p);
+            const double p = As[j * n + j];           // unscaled pivot
+            if (!(p > 0.0)) { failed = true; break; } // the same for every thread
+            if (j > 0) {                              // scale column j - 1
+                const double sq = sqrt(pprev);
+                const double inv = 1.0 / sq;
+                const int i = j - 1 + tid;
+                if (i == j - 1) As[(j - 1) * n + (j - 1)] = sq;
+                else if (i < n) As[i * n + (j - 1)] *= inv;
             }
-            __syncthreads();
-            if (fail) break;
-            const double ljj = As[j * n + j];
-            for (int i = j + 1 + tid; i < n; i += nt) As[i * n + j] /= ljj;
-            __syncthreads();
-            const int m = n - j - 1;
-            const int pairs = m * (m + 1) / 2;
-            for (int e = tid; e < pairs; e += nt) {
-                int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-                while ((r + 1) * (r + 2) / 2 <= e) ++r;
-                while (r * (r + 1) / 2 > e) --r;
-                const int cc = e - r * (r + 1) / 2;
-                const int i = j + 1 + r, k = j + 1 + cc;
-                As[i * n + k] = fma(-As[i * n + j], As[k * n + j], As[i * n + k]);
+            const double rp = 1.0 / p;
+            for (int i = j + 1 + ty; i < n; i += 32) {
+                const double aij = As[i * n + j] * rp;
+                for (int k = j + 1 + tx; k <= i; k += 32)
+                    As[i * n + k] = fma(-aij, As[k * n + j], As[i * n + k]);
             }
+            pprev = p;
             __syncthreads();
         }
-        ok = !fail;
+        if (!failed) {                                // scale the last column
+            if (tid == 0) As[(n - 1) * n + (n - 1)] = sqrt(pprev);
+            __syncthreads();
+        }
+        ok = !failed;
         __syncthreads();
     }
     if (tid == 0) {
@@ -250,8 +288,10 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
 }
 
 // W[:, c0:c0+32] from L y = b, L^T x = y with L and the 32 right-hand sides
-// in LDS (n <= 126).  Forward and backward substitution are right-looking
-// over the 32 columns in parallel.
+// in LDS (n <= 126).  One barrier per row: every thread derives x_i of its
+// column from the still unreduced row i and updates the rows after (before)
+// it; the thread owning column c writes x_i back one step later, when no step
+// reads row i any more.
 __global__ __launch_bounds__(256) void k_trsm_lds(const double *__restrict__ L, int n,
                                                   const double *__restrict__ B, int nrhs,
                                                   double *__restrict__ W,
@@ -259,7 +299,9 @@ __global__ __launch_bounds__(256) void k_trsm_lds(const double *__restrict__ L, 
     if (scal[kScalSolveOK] != 1.0) return;
     __shared__ double Ls[kLdsN * kLdsN];
     __shared__ double Xs[kLdsN * 32];
+    __shared__ double rd[kLdsN];
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int c = tid & 31, pr = tid >> 5;           // column, row phase (8 phases)
     const int c0 = blockIdx.x * 32;
     const int nc = (nrhs - c0 < 32) ? nrhs - c0 : 32;
     for (int e = tid; e < n * n; e += nt) Ls[e] = L[e];
@@ -267,40 +309,79 @@ __global__ __launch_bounds__(256) void k_trsm_lds(const double *__restrict__ L, 
         const int i = e / 32, cc = e % 32;
         Xs[e] = (cc < nc) ? B[(long)i * nrhs + c0 + cc] : 0.0;
     }
+    for (int i = tid; i < n; i += nt) rd[i] = 1.0 / L[(long)i * n + i];
     __syncthreads();
+    double xprev = 0.0;
     for (int i = 0; i < n; ++i) {                  // forward
-        if (tid < 32) Xs[i * 32 + tid] /= Ls[i * n + i];
-        __syncthreads();
-        for (int e = tid; e < (n - i - 1) * 32; e += nt) {
-            const int p = i + 1 + e / 32, cc = e % 32;
-            Xs[p * 32 + cc] = fma(-Ls[p * n + i], Xs[i * 32 + cc], Xs[p * 32 + cc]);
-        }
+        if (i > 0 && tid < 32) Xs[(i - 1) * 32 + c] = xprev;
+        const double xi = Xs[i * 32 + c] * rd[i];
+        for (int p = i + 1 + pr; p < n; p += 8)
+            Xs[p * 32 + c] = fma(-Ls[p * n + i], xi, Xs[p * 32 + c]);
+        xprev = xi;
         __syncthreads();
     }
+    if (tid < 32) Xs[(n - 1) * 32 + c] = xprev;
+    __syncthreads();
     for (int i = n - 1; i >= 0; --i) {             // backward with L^T
-        if (tid < 32) Xs[i * 32 + tid] /= Ls[i * n + i];
-        __syncthreads();
-        for (int e = tid; e < i * 32; e += nt) {
-            const int p = e / 32, cc = e % 32;
-            Xs[p * 32 + cc] = fma(-Ls[i * n + p], Xs[i * 32 + cc], Xs[p * 32 + cc]);
-        }
+        if (i < n - 1 && tid < 32) Xs[(i + 1) * 32 + c] = xprev;
+        const double xi = Xs[i * 32 + c] * rd[i];
+        for (int p = pr; p < i; p += 8)
+            Xs[p * 32 + c] = fma(-Ls[i * n + p], xi, Xs[p * 32 + c]);
+        xprev = xi;
         __syncthreads();
     }
+    if (tid < 32) Xs[c] = xprev;
+    __syncthreads();
     for (int e = tid; e < n * 32; e += nt) {
         const int i = e / 32, cc = e % 32;
         if (cc < nc) W[(long)i * nrhs + c0 + cc] = Xs[e];
     }
 }
 
-// Y[k][d] = sum_i FI[k][i] W[i][d] into the padded FP64 layout.
-__global__ void k_phiw(const double *__restrict__ FI, const double *__restrict__ W, int K,
-                       int M1, int D, int ld, double *__restrict__ Y) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (d >= D) return;
-    double s = 0.0;
-    for (int i = 0; i < M1; ++i) s = fma(FI[(long)k * M1 + i], W[(long)i * D + d], s);
-    Y[(long)k * ld + d] = s;
+// Y = FI W into the padded FP64 layout (K x ld, columns >= D untouched):
+// 64 x 64 output tiles over (centre, d), the M1 inner dimension in steps of 16.
+__global__ __launch_bounds__(256) void k_phiw(const double *__restrict__ FI,
+                                              const double *__restrict__ W, int K, int M1, int D,
+                                              int ld, double *__restrict__ Y) {
+    __shared__ double As[GK][GT + 1], Bs[GK][GT];
+    const int k0 = blockIdx.x * GT, d0 = blockIdx.y * GT;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int ib = 0; ib < M1; ib += GK) {
+        for (int e = tid; e < GK * GT; e += 256) {
+            const int row = e / GK, ii = e % GK;          // FI rows read along i
+            const int k = k0 + row, i = ib + ii;
+            As[ii][row] = (k < K && i < M1) ? FI[(long)k * M1 + i] : 0.0;
+            const int kk = e / GT, col = e % GT;          // W rows read along d
+            const int i2 = ib + kk, d = d0 + col;
+            Bs[kk][col] = (i2 < M1 && d < D) ? W[(long)i2 * D + d] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int k = k0 + ty + 16 * a, d = d0 + tx + 16 * b;
+            if (k < K && d < D) Y[(long)k * ld + d] = acc[a][b];
+        }
 }
 
 // Per-workgroup partials of sum_k (G_k |y_k|^2 - 2 y_k . RT_k) over 256
@@ -366,8 +447,10 @@ int dg_mstep_run(dg_ctx *c) {
     const long E = (long)M1 * M1 + (long)M1 * D;
     const int nch = c->nchAB;
     const int kc = (K + nch - 1) / nch;
-    hipLaunchKernelGGL(k_partAB, dim3((unsigned)((E + 255) / 256), (unsigned)nch), dim3(256), 0,
-                       c->stream, c->FI, c->G, c->RT, K, M1, D, kc, c->partAB);
+    hipLaunchKernelGGL(k_partAB,
+                       dim3((unsigned)((M1 + GT - 1) / GT), (unsigned)((M1 + D + GT - 1) / GT),
+                            (unsigned)nch),
+                       dim3(256), 0, c->stream, c->FI, c->G, c->RT, K, M1, D, kc, c->partAB);
     DG_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_sumAB, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, c->stream,
                        c->partAB, nch, M1, D, c->A0, c->Bm);
@@ -384,8 +467,8 @@ int dg_mstep_run(dg_ctx *c) {
                            ridge, c->Bm, D, c->W, c->scal);
     }
     DG_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_phiw, dim3((unsigned)((D + 63) / 64), (unsigned)K), dim3(64), 0,
-                       c->stream, c->FI, c->W, K, M1, D, c->ldt, c->Y64);
+    hipLaunchKernelGGL(k_phiw, dim3((unsigned)((K + GT - 1) / GT), (unsigned)((D + GT - 1) / GT)),
+                       dim3(256), 0, c->stream, c->FI, c->W, K, M1, D, c->ldt, c->Y64);
     DG_HIP(c, hipGetLastError());
     int rc = dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy);
     if (rc) return rc;
