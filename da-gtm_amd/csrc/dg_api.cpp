@@ -176,7 +176,7 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
         (rc = dalloc(c, &c->logh, N)) || (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
         (rc = dalloc(c, &c->partG, (size_t)c->NC * K)) ||
         (rc = dalloc(c, &c->partAB, std::max((size_t)c->nchAB * ((size_t)M1 * M1 + (size_t)M1 * D),
-                                             (size_t)(K + 255) / 256)))) {
+                                             (size_t)(K + 63) / 64)))) {
         free_problem(c);
         return rc;
     }

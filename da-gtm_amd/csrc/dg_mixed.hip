@@ -871,15 +871,19 @@ __global__ void k_transpose16(const _Float16 *__restrict__ T16, long N, long Np,
 // Split Y (K x D, stride ldt, FP64) into two FP16 terms with a power-of-two
 // scale chosen from scal_in[0] = max |Y|: Y16[t][k][d], rows k >= K zero;
 // yy32[k] = |y'|^2 of the split value; mscal[0] = the scale.
-__global__ void k_make_y16(const double *__restrict__ Y64, int ldt, int K, int Kp, int D, int ld16,
-                           const double *__restrict__ amax, _Float16 *__restrict__ Y16,
-                           float *__restrict__ yy32, double *__restrict__ mscal) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_make_y16(const double *__restrict__ Y64, int ldt, int K,
+                                                  int Kp, int D, int ld16,
+                                                  const double *__restrict__ amax,
+                                                  _Float16 *__restrict__ Y16,
+                                                  float *__restrict__ yy32,
+                                                  double *__restrict__ mscal) {
+    // one wave per centre row; lanes step over d, the norm is a wave sum
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
     const double scale = pow2_scale(amax[0]);
-    if (k == 0) mscal[0] = scale;
+    if (blockIdx.x == 0 && threadIdx.x == 0) mscal[0] = scale;
     if (k >= Kp) return;
     double ss = 0.0;
-    for (int d = 0; d < ld16; ++d) {
+    for (int d = l; d < ld16; d += 64) {
         const double v = (k < K && d < D) ? Y64[(long)k * ldt + d] * scale : 0.0;
         const _Float16 hi = (_Float16)(float)v;
         const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
@@ -888,7 +892,9 @@ __global__ void k_make_y16(const double *__restrict__ Y64, int ldt, int K, int K
         const double rep = ((double)(float)hi + (double)(float)lo) / scale;
         ss = fma(rep, rep, ss);
     }
-    yy32[k] = (float)ss;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+    if (l == 0) yy32[k] = (float)ss;
 }
 
 // Supported numbers of 16-wide d steps (D is padded up to the next one).
@@ -1104,7 +1110,7 @@ int dg_mixed_prepare_y(dg_ctx *c) {
     hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, c->stream, b->amaxPart, nwg,
                        c->scal + 6);
     DG_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_make_y16, dim3((unsigned)((b->Kp + 255) / 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_make_y16, dim3((unsigned)((b->Kp + 3) / 4)), dim3(256), 0, c->stream,
                        c->Y64, c->ldt, c->K, b->Kp, c->D, c->ldh, c->scal + 6, b->Y16, c->yy32,
                        b->mscal);
     DG_HIP(c, hipGetLastError());

@@ -223,19 +223,23 @@ __global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
 }
 
 constexpr int kLdsN = 126;   // largest M+1 whose factor fits in LDS
+static_assert(kLdsN <= 128, "the Cholesky and solve kernels give each thread at most 4 rows");
 
-// Cholesky of A0 + (ridge + jitter) I with the matrix resident in LDS (one
-// workgroup, n <= 126); same pivots and retries as k_chol_solve.  One barrier
-// per column: step j updates the trailing lower triangle with the column
-// still unscaled, A[i][k] -= A[i][j] A[k][j] / p_j, and scales column j - 1,
-// which no step reads any more.  Writes the factor's lower triangle to L.
+// Cholesky of A0 + (ridge + jitter) I by one workgroup (n <= 126), same
+// pivots and retries as k_chol_solve.  Thread (tx, ty) of 32 x 32 keeps the
+// lower-triangle elements A[i][k], i = ty + 32 a, k = tx + 32 b, in registers
+// for the whole factorization.  Step j reads the still unscaled pivot column
+// j from an LDS ping-pong buffer, writes column j of L, updates its elements
+// (A[i][k] -= A[i][j] A[k][j] / A[j][j]) and the owners of column j + 1 publish
+// it in the other buffer: one barrier per step, no LDS read-modify-write.
+// Writes the lower triangle of L.
 __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
                                                    double *__restrict__ L, int n, double ridge,
                                                    double *__restrict__ scal) {
-    __shared__ double As[kLdsN * kLdsN];
+    __shared__ double col[2][128];
     __shared__ double tr;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int tx = tid & 31, ty = tid >> 5;          // 32 x 32 threads over (row, column)
+    const int tid = threadIdx.x;
+    const int tx = tid & 31, ty = tid >> 5;
     if (tid == 0) {
         double s = 0.0;
         for (int i = 0; i < n; ++i) s += A0[(long)i * n + i];
@@ -247,35 +251,62 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
     for (int attempt = 0; attempt < 9 && !ok; ++attempt) {
         if (attempt == 1) jit = 1e-13 * tr / n;
         else if (attempt > 1) jit *= 10.0;
-        for (int e = tid; e < n * n; e += nt) {
-            const int i = e / n, j = e % n;
-            As[e] = A0[e] + ((i == j) ? ridge + jit : 0.0);
+        double v[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = ty + 32 * a, k = tx + 32 * b;
+                v[a][b] = (i < n && k <= i) ? A0[(long)i * n + k] + ((i == k) ? ridge + jit : 0.0)
+                                            : 0.0;
+            }
+        // publish column 0
+        if (tx == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = ty + 32 * a;
+                if (i < n) col[0][i] = v[a][0];
+            }
         }
-        __syncthreads();
         bool failed = false;
-        double pprev = 0.0;
         for (int j = 0; j < n; ++j) {
-            const double p = As[j * n + j];           // unscaled pivot
-            if (!(p > 0.0)) { failed = true; break; } // the same for every thread
-            if (j > 0) {                              // scale column j - 1
-                const double sq = sqrt(pprev);
-                const double inv = 1.0 / sq;
-                const int i = j - 1 + tid;
-                if (i == j - 1) As[(j - 1) * n + (j - 1)] = sq;
-                else if (i < n) As[i * n + (j - 1)] *= inv;
-            }
+            __syncthreads();
+            const double *cj = col[j & 1];
+            const double p = cj[j];
+            if (!(p > 0.0)) { failed = true; break; }   // the same for every thread
             const double rp = 1.0 / p;
-            for (int i = j + 1 + ty; i < n; i += 32) {
-                const double aij = As[i * n + j] * rp;
-                for (int k = j + 1 + tx; k <= i; k += 32)
-                    As[i * n + k] = fma(-aij, As[k * n + j], As[i * n + k]);
+            if (tx == (j & 31)) {                        // column j of L
+                const double sq = sqrt(p), inv = 1.0 / sq;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = ty + 32 * a;
+                    if (i == j) L[(long)j * n + j] = sq;
+                    else if (i > j && i < n) L[(long)i * n + j] = cj[i] * inv;
+                }
             }
-            pprev = p;
-            __syncthreads();
-        }
-        if (!failed) {                                // scale the last column
-            if (tid == 0) As[(n - 1) * n + (n - 1)] = sqrt(pprev);
-            __syncthreads();
+            double ci[4], ck[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = ty + 32 * a, k = tx + 32 * a;
+                ci[a] = (i > j && i < n) ? cj[i] * rp : 0.0;
+                ck[a] = (k > j && k < n) ? cj[k] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) v[a][b] = fma(-ci[a], ck[b], v[a][b]);
+            // owners of column j + 1 publish it (unscaled) for the next step
+            if (j + 1 < n && tx == ((j + 1) & 31)) {
+                const int bj = (j + 1) >> 5;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = ty + 32 * a;
+                    double val = v[a][0];
+#pragma unroll
+                    for (int b = 1; b < 4; ++b) val = (bj == b) ? v[a][b] : val;
+                    if (i > j && i < n) col[(j + 1) & 1][i] = val;
+                }
+            }
         }
         ok = !failed;
         __syncthreads();
@@ -284,57 +315,80 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
         scal[kScalJitter] = jit;
         scal[kScalSolveOK] = ok ? 1.0 : 0.0;
     }
-    for (int e = tid; e < n * n; e += nt) L[e] = As[e];
 }
 
-// W[:, c0:c0+32] from L y = b, L^T x = y with L and the 32 right-hand sides
-// in LDS (n <= 126).  One barrier per row: every thread derives x_i of its
-// column from the still unreduced row i and updates the rows after (before)
-// it; the thread owning column c writes x_i back one step later, when no step
-// reads row i any more.
-__global__ __launch_bounds__(256) void k_trsm_lds(const double *__restrict__ L, int n,
-                                                  const double *__restrict__ B, int nrhs,
-                                                  double *__restrict__ W,
-                                                  const double *__restrict__ scal) {
+// W[:, c0:c0+32] from L y = b, L^T x = y (n <= 126): L in LDS, thread
+// (c, pr) of 32 x 32 keeps rows p = pr + 32 a of right-hand side c in
+// registers.  Step i: the owner of row i finishes x_i and publishes it in an
+// LDS ping-pong buffer; after one barrier every thread updates its rows after
+// (forward) or before (backward) row i.
+__global__ __launch_bounds__(1024) void k_trsm_lds(const double *__restrict__ L, int n,
+                                                   const double *__restrict__ B, int nrhs,
+                                                   double *__restrict__ W,
+                                                   const double *__restrict__ scal) {
     if (scal[kScalSolveOK] != 1.0) return;
     __shared__ double Ls[kLdsN * kLdsN];
-    __shared__ double Xs[kLdsN * 32];
     __shared__ double rd[kLdsN];
+    __shared__ double xb[2][32];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int c = tid & 31, pr = tid >> 5;           // column, row phase (8 phases)
+    const int c = tid & 31, pr = tid >> 5;
     const int c0 = blockIdx.x * 32;
-    const int nc = (nrhs - c0 < 32) ? nrhs - c0 : 32;
-    for (int e = tid; e < n * n; e += nt) Ls[e] = L[e];
-    for (int e = tid; e < n * 32; e += nt) {
-        const int i = e / 32, cc = e % 32;
-        Xs[e] = (cc < nc) ? B[(long)i * nrhs + c0 + cc] : 0.0;
+    const bool live = c0 + c < nrhs;
+    for (int e = tid; e < n * n; e += nt) {
+        const int i = e / n, k = e % n;
+        if (k <= i) Ls[e] = L[e];                    // lower triangle only
     }
     for (int i = tid; i < n; i += nt) rd[i] = 1.0 / L[(long)i * n + i];
+    double x[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int p = pr + 32 * a;
+        x[a] = (live && p < n) ? B[(long)p * nrhs + c0 + c] : 0.0;
+    }
     __syncthreads();
-    double xprev = 0.0;
     for (int i = 0; i < n; ++i) {                  // forward
-        if (i > 0 && tid < 32) Xs[(i - 1) * 32 + c] = xprev;
-        const double xi = Xs[i * 32 + c] * rd[i];
-        for (int p = i + 1 + pr; p < n; p += 8)
-            Xs[p * 32 + c] = fma(-Ls[p * n + i], xi, Xs[p * 32 + c]);
-        xprev = xi;
+        if (pr == (i & 31)) {
+            const int ai = i >> 5;
+            double xi = x[0];
+#pragma unroll
+            for (int a = 1; a < 4; ++a) xi = (ai == a) ? x[a] : xi;
+            xi *= rd[i];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) x[a] = (ai == a) ? xi : x[a];
+            xb[i & 1][c] = xi;
+        }
         __syncthreads();
+        const double xi = xb[i & 1][c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int p = pr + 32 * a;
+            if (p > i && p < n) x[a] = fma(-Ls[p * n + i], xi, x[a]);
+        }
     }
-    if (tid < 32) Xs[(n - 1) * 32 + c] = xprev;
-    __syncthreads();
+    __syncthreads();                               // xb is reused from here on
     for (int i = n - 1; i >= 0; --i) {             // backward with L^T
-        if (i < n - 1 && tid < 32) Xs[(i + 1) * 32 + c] = xprev;
-        const double xi = Xs[i * 32 + c] * rd[i];
-        for (int p = pr; p < i; p += 8)
-            Xs[p * 32 + c] = fma(-Ls[i * n + p], xi, Xs[p * 32 + c]);
-        xprev = xi;
+        if (pr == (i & 31)) {
+            const int ai = i >> 5;
+            double xi = x[0];
+#pragma unroll
+            for (int a = 1; a < 4; ++a) xi = (ai == a) ? x[a] : xi;
+            xi *= rd[i];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) x[a] = (ai == a) ? xi : x[a];
+            xb[i & 1][c] = xi;
+        }
         __syncthreads();
+        const double xi = xb[i & 1][c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int p = pr + 32 * a;
+            if (p < i) x[a] = fma(-Ls[i * n + p], xi, x[a]);
+        }
     }
-    if (tid < 32) Xs[c] = xprev;
-    __syncthreads();
-    for (int e = tid; e < n * 32; e += nt) {
-        const int i = e / 32, cc = e % 32;
-        if (cc < nc) W[(long)i * nrhs + c0 + cc] = Xs[e];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int p = pr + 32 * a;
+        if (live && p < n) W[(long)p * nrhs + c0 + c] = x[a];
     }
 }
 
@@ -384,37 +438,44 @@ __global__ __launch_bounds__(256) void k_phiw(const double *__restrict__ FI,
         }
 }
 
-// Per-workgroup partials of sum_k (G_k |y_k|^2 - 2 y_k . RT_k) over 256
-// centres each; k_beta_final sums them in workgroup order into
-// scal[kScalBetaSum].
+// Per-workgroup partials of sum_k (G_k |y_k|^2 - 2 y_k . RT_k) over 64
+// centres each (one wave per centre, 16 centres per wave); k_beta_final sums
+// them in workgroup order into scal[kScalBetaSum].
+constexpr int kBetaWG = 64;
 __global__ __launch_bounds__(256) void k_beta_part(const double *__restrict__ Y, int ld,
                                                    const double *__restrict__ yy,
                                                    const double *__restrict__ G,
                                                    const double *__restrict__ RT, int K, int D,
                                                    double *__restrict__ part) {
-    __shared__ double s[256];
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    double acc = 0.0;
-    if (k < K) {
+    __shared__ double s[4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double acc = 0.0;                              // lane 0: this wave's sum
+    for (int q = 0; q < kBetaWG / 4; ++q) {
+        const int k = blockIdx.x * kBetaWG + w * (kBetaWG / 4) + q;
+        if (k >= K) break;
         double dot = 0.0;
-        for (int d = 0; d < D; ++d) dot = fma(Y[(long)k * ld + d], RT[(long)k * D + d], dot);
-        acc = G[k] * yy[k] - 2.0 * dot;
+        for (int d = l; d < D; d += 64) dot = fma(Y[(long)k * ld + d], RT[(long)k * D + d], dot);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+        acc += G[k] * yy[k] - 2.0 * dot;
     }
-    s[threadIdx.x] = acc;
+    if (l == 0) s[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ part, int n,
+                                                    double *__restrict__ scal) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += part[i];
+    s[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-}
-
-__global__ void k_beta_final(const double *__restrict__ part, int n, double *__restrict__ scal) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += part[i];
-        scal[kScalBetaSum] = s;
-    }
+    if (threadIdx.x == 0) scal[kScalBetaSum] = s[0];
 }
 
 }  // namespace
@@ -460,7 +521,7 @@ int dg_mstep_run(dg_ctx *c) {
         hipLaunchKernelGGL(k_chol_lds, dim3(1), dim3(1024), 0, c->stream, c->A0, c->A, M1, ridge,
                            c->scal);
         DG_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + 31) / 32)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + 31) / 32)), dim3(1024), 0, c->stream,
                            c->A, M1, c->Bm, D, c->W, c->scal);
     } else {
         hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, c->stream, c->A0, c->A, M1,
@@ -473,11 +534,11 @@ int dg_mstep_run(dg_ctx *c) {
     int rc = dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy);
     if (rc) return rc;
     // beta partials reuse the (now consumed) A/B partial buffer
-    const int nb = (K + 255) / 256;
+    const int nb = (K + kBetaWG - 1) / kBetaWG;
     hipLaunchKernelGGL(k_beta_part, dim3((unsigned)nb), dim3(256), 0, c->stream, c->Y64, c->ldt,
                        c->yy, c->G, c->RT, K, D, c->partAB);
     DG_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_beta_final, dim3(1), dim3(64), 0, c->stream, c->partAB, nb, c->scal);
+    hipLaunchKernelGGL(k_beta_final, dim3(1), dim3(256), 0, c->stream, c->partAB, nb, c->scal);
     DG_HIP(c, hipGetLastError());
     return DG_OK;
 }
