@@ -7,8 +7,10 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dg_internal.h"
@@ -50,7 +52,7 @@ void dfree(T *&p) {
 
 void free_problem(dg_ctx *c) {
     dfree(c->T64); dfree(c->tt); dfree(c->FI); dfree(c->Y64); dfree(c->yy);
-    dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm); dfree(c->G); dfree(c->RT);
+    dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm); dfree(c->G); dfree(c->Gm); dfree(c->RT);
     dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
     dfree(c->partRT); dfree(c->partG); dfree(c->partAB);
     dg_mixed_free(c);
@@ -171,6 +173,7 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
         (rc = dalloc(c, &c->yy, K)) || (rc = dalloc(c, &c->W, (size_t)M1 * D)) ||
         (rc = dalloc(c, &c->A, (size_t)M1 * M1)) || (rc = dalloc(c, &c->A0, (size_t)M1 * M1)) ||
         (rc = dalloc(c, &c->Bm, (size_t)M1 * D)) || (rc = dalloc(c, &c->G, K)) ||
+        (rc = dalloc(c, &c->Gm, K)) ||
         (rc = dalloc(c, &c->RT, (size_t)K * D)) || (rc = dalloc(c, &c->pm, N)) ||
         (rc = dalloc(c, &c->pl, N)) || (rc = dalloc(c, &c->logg, N)) ||
         (rc = dalloc(c, &c->logh, N)) || (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
@@ -182,6 +185,7 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     }
     DG_HIP(c, hipMemsetAsync(c->T64, 0, (size_t)N * ld * sizeof(double), c->stream));
     DG_HIP(c, hipMemsetAsync(c->Y64, 0, (size_t)K * ld * sizeof(double), c->stream));
+    DG_HIP(c, hipMemsetAsync(c->Gm, 0, (size_t)K * sizeof(double), c->stream));
     DG_HIP(c, hipMemcpy2DAsync(c->T64, ld * sizeof(double), T, (size_t)ldT * sizeof(double),
                                (size_t)D * sizeof(double), (size_t)N, hipMemcpyHostToDevice,
                                c->stream));
@@ -214,6 +218,7 @@ int dg_set_state(dg_ctx *c, const double *Y, long ldY, double beta) {
                                (size_t)ldY * sizeof(double), (size_t)c->D * sizeof(double),
                                (size_t)c->K, hipMemcpyHostToDevice, c->stream));
     if ((rc = dg_rownorms(c, c->Y64, c->K, c->ldt, c->D, c->yy))) return rc;
+    DG_HIP(c, hipMemsetAsync(c->Gm, 0, (size_t)c->K * sizeof(double), c->stream));
     c->beta = beta;
     c->have_estep = false;
     if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
@@ -256,6 +261,10 @@ int dg_mstep(dg_ctx *c, double *beta_out) {
     if (!c->loaded || !c->have_estep) return dg_fail(c, DG_ESTATE, "dg_mstep needs a dg_estep first");
     int rc;
     if ((rc = select_device(c))) return rc;
+    // the G of this M-step is the one the critical-temperature search reads
+    // (vggsub of dagtm.cxx:1715, still set when the next E-step converges)
+    DG_HIP(c, hipMemcpyAsync(c->Gm, c->G, (size_t)c->K * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[4], c->stream));
     if ((rc = dg_mstep_run(c))) return rc;
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[5], c->stream));
@@ -279,6 +288,9 @@ int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_it
               void *user) {
     if (!c || !p || !res) return DG_EINVAL;
     if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_anneal before dg_upload");
+    if (p->schedule != DG_SCHED_EXP && p->schedule != DG_SCHED_LINEAR &&
+        p->schedule != DG_SCHED_AUTO)
+        return dg_fail(c, DG_EINVAL, "dg_anneal: unknown schedule");
     const int QN = 50;   // INIT_QUEUE_SIZE, dagtm.cxx:45
     double Q[QN];
     int qnext = 0;
@@ -314,14 +326,112 @@ int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_it
             if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
         }
         if (p->isDA && !isLast) {
-            if (p->schedule == DG_SCHED_LINEAR) temp -= p->delta;
-            else temp *= p->alpha;
+            if (p->schedule == DG_SCHED_LINEAR) {
+                temp -= p->delta;
+            } else if (p->schedule == DG_SCHED_AUTO) {
+                // dagtm.cxx:1919-1935: the largest critical temperature
+                // below the current one, else one EXP step
+                double ct = 0.0;
+                if ((rc = dg_next_ctemp(c, temp, p->eps, nullptr, &ct, nullptr))) return rc;
+                temp = (ct >= temp) ? temp * p->alpha : ct;
+            } else {
+                temp *= p->alpha;
+            }
         } else {
             break;
         }
     } while (!isLast);
     res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
     res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual; res->beta = c->beta;
+    return DG_OK;
+}
+
+int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *ctemp_max,
+                  double *ctemp_k) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded || !c->have_estep)
+        return dg_fail(c, DG_ESTATE, "dg_next_ctemp needs a dg_estep at the current state");
+    if (!ctemp_max || !std::isfinite(temp) || !std::isfinite(eps))
+        return dg_fail(c, DG_EINVAL, "dg_next_ctemp: bad args");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    // R in FP64: the mixed path refreshes the FP64 per-point normalisers
+    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+        return rc;
+    const int K = c->K, D = c->D;
+    const long DD = (long)D * D;
+    std::vector<double> g(K);
+    if (gg) {
+        std::copy(gg, gg + K, g.begin());
+    } else {
+        DG_HIP(c, hipMemcpyAsync(g.data(), c->Gm, (size_t)K * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        DG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    // centres per batch: S1 and S2 of a batch take at most 256 MB
+    const int KB = (int)std::max(1L, std::min((long)K, (1L << 25) / (2 * DD)));
+    const int nt = (D + 63) / 64, pairs = nt * (nt + 1) / 2;
+    const long tiles = (c->N + 63) / 64;
+    long nc = (2048 + (long)pairs * KB - 1) / ((long)pairs * KB);
+    nc = std::max(1L, std::min({nc, 64L, tiles}));
+    const size_t slen = (size_t)KB * 2 * DD;
+    double *dPart = nullptr, *dS = nullptr;
+    if ((rc = dalloc(c, &dPart, slen * nc))) return rc;
+    if (nc > 1 && (rc = dalloc(c, &dS, slen))) { dfree(dPart); return rc; }
+    std::vector<double> hS(slen), ck(K);
+    const double GG_MIN = 1e-250;   // mpicomm.c:617-628
+    int bad = 0;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (int k0 = 0; k0 < K && !rc; k0 += KB) {
+        const int kb = std::min(KB, K - k0);
+        rc = dg_ctemp_moments_fp64(c, k0, kb, (int)nc, dPart, nc > 1 ? dS : dPart);
+        hipError_t e = hipSuccess;
+        if (!rc) e = hipMemcpyAsync(hS.data(), nc > 1 ? dS : dPart, (size_t)kb * 2 * DD * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream);
+        if (!rc && e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = dg_hip_check(c, e, "dg_next_ctemp");
+        if (rc) break;
+        // per centre: eig(H) of mpicomm.c:599-626 with H = [[2S1 - S2, -S2], [-S2, 2S1 - S2]]
+        // is eig(2 S1) together with eig(2 S1 - 2 S2) (H is block-symmetric)
+        std::atomic<int> fail{0};
+        auto work = [&](unsigned t) {
+            std::vector<double> A((size_t)DD), w(D);
+            for (int i = (int)t; i < kb; i += (int)nth) {
+                const int k = k0 + i;
+                if (g[k] == 0.0) { ck[k] = HUGE_VAL; continue; }
+                const double *S1 = hS.data() + (size_t)i * 2 * DD, *S2 = S1 + DD;
+                const double sc = g[k] < GG_MIN ? 1.0 / GG_MIN : 1.0;
+                const double limit = (temp - eps) * g[k] * 2.0 / c->beta;
+                double lmax = -HUGE_VAL;
+                for (int part = 0; part < 2; ++part) {
+                    for (int a = 0; a < D; ++a)
+                        for (int b = 0; b <= a; ++b) {
+                            const double s1 = 2.0 * S1[(size_t)a * D + b];
+                            const double v = part ? s1 - 2.0 * S2[(size_t)a * D + b] : s1;
+                            A[(size_t)a * D + b] = v * sc;
+                        }
+                    if (dg_sym_eigvals(D, A.data(), D, w.data())) { fail = 1; return; }
+                    for (int j = 0; j < D; ++j) {
+                        const double l = (w[j] / sc >= limit) ? 0.0 : w[j] / sc;   // comm.c:2149-2161
+                        if (l > lmax) lmax = l;
+                    }
+                }
+                ck[k] = c->beta / 2.0 / g[k] * lmax;
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nth; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto &th : pool) th.join();
+        bad |= fail.load();
+    }
+    dfree(dPart); dfree(dS);
+    if (rc) return rc;
+    if (bad) return dg_fail(c, DG_ESOLVE, "dg_next_ctemp: eigenvalue iteration did not converge");
+    double cmax = -HUGE_VAL;
+    for (int k = 0; k < K; ++k) cmax = std::max(cmax, ck[k]);
+    *ctemp_max = cmax;
+    if (ctemp_k) std::copy(ck.begin(), ck.end(), ctemp_k);
     return DG_OK;
 }
 

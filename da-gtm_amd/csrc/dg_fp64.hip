@@ -376,6 +376,110 @@ int launch_e2(dg_ctx *c, double nbh, double it, long chunk, dim3 grid) {
     return DG_OK;
 }
 
+// ---------------------------------------------------------------- critical temperature
+// Responsibility-weighted second moments of the data centred on one centre,
+// S1 = sum_n r_kn x x^T and S2 = sum_n r_kn^2 x x^T with x = t_n - y_k (the
+// 2 S1 and -S2 of mpicomm.c:580-597).  One workgroup per (64 x 64 tile of the
+// lower triangle, centre k0 + blockIdx.y, N chunk); R_kn is recomputed with
+// k_resp_dense's formula.  Each thread owns a 4 x 4 micro-tile of both sums,
+// accumulated over the chunk's points in order.  The chunk's partial goes to
+// part[((chunk * KB + kb) * 2 + s) * D * D + a * D + b] for the tile only.
+constexpr int CM = 64;   // moment tile edge
+
+__global__ __launch_bounds__(NT) void k_ctemp_moments(
+    const double *__restrict__ T, const double *__restrict__ tt, long N,
+    const double *__restrict__ Y, const double *__restrict__ yy, int k0, int D, int ld,
+    double nbh, double it, double invK, const double *__restrict__ pm,
+    const double *__restrict__ pl, long chunk, double *__restrict__ part) {
+    __shared__ double Xa[CM][CM + 1];
+    __shared__ double Xb[CM][CM + 1];
+    __shared__ double Ys[192];
+    __shared__ double rs[CM];
+    const int tid = threadIdx.x;
+    int p = blockIdx.x, ti = 0;            // lower-triangle pair p -> (ti >= tj)
+    while (p > ti) { p -= ti + 1; ++ti; }
+    const int tj = p;
+    const int kb = blockIdx.y, k = k0 + kb;
+    const long n0 = (long)blockIdx.z * chunk;
+    const long n1 = min(N, n0 + chunk);
+    const int a0 = ti * CM, b0 = tj * CM;
+    const int tx = tid & 15, ty = tid >> 4;
+    for (int d = tid; d < ld; d += NT) Ys[d] = Y[(long)k * ld + d];
+    double s1[4][4], s2[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s1[i][j] = s2[i][j] = 0.0;
+    __syncthreads();
+    for (long base = n0; base < n1; base += CM) {
+        if (tid < CM) {
+            const long n = base + tid;
+            double r = 0.0;
+            if (n < n1) {
+                if (pm[n] == -INFINITY) {
+                    r = invK;
+                } else {
+                    double s = 0.0;
+                    for (int d = 0; d < ld; ++d) s = fma(T[n * ld + d], Ys[d], s);
+                    const double x = exponent(s, yy[k], tt[n], nbh, true);
+                    r = (x != -INFINITY) ? exp((x - pm[n]) * it - pl[n]) : 0.0;
+                }
+            }
+            rs[tid] = r;
+        }
+        for (int e = tid; e < CM * CM; e += NT) {
+            const int q = e / CM, dd = e % CM;
+            const long n = base + q;
+            const int a = a0 + dd, b = b0 + dd;
+            Xa[q][dd] = (n < n1 && a < D) ? T[n * ld + a] - Ys[a] : 0.0;
+            Xb[q][dd] = (n < n1 && b < D) ? T[n * ld + b] - Ys[b] : 0.0;
+        }
+        __syncthreads();
+        const int np = (int)min((long)CM, n1 - base);
+        for (int q = 0; q < np; ++q) {
+            const double r = rs[q];
+            double ra[4], xb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ra[i] = Xa[q][tx + 16 * i] * r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xb[j] = Xb[q][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double rra = ra[i] * r;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    s1[i][j] = fma(ra[i], xb[j], s1[i][j]);
+                    s2[i][j] = fma(rra, xb[j], s2[i][j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const long DD = (long)D * D;
+    double *o1 = part + ((long)blockIdx.z * gridDim.y + kb) * 2 * DD;
+    double *o2 = o1 + DD;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int a = a0 + tx + 16 * i, b = b0 + ty + 16 * j;
+            if (a < D && b < D) {
+                o1[(long)a * D + b] = s1[i][j];
+                o2[(long)a * D + b] = s2[i][j];
+            }
+        }
+}
+
+// out[i] = sum over the nc chunk partials of part[c * len + i], in chunk order.
+__global__ void k_sum_chunks(const double *__restrict__ part, int nc, long len,
+                             double *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int c = 0; c < nc; ++c) s += part[(long)c * len + i];
+    out[i] = s;
+}
+
 }  // namespace
 
 int dg_estep_fp64(dg_ctx *c, int isDA, double temp) {
@@ -429,6 +533,25 @@ int dg_resp_dense_fp64(dg_ctx *c, double *dR, long ldR) {
                        c->yy, c->K, c->ldt, -c->beta / 2.0, it, 1.0 / c->K, c->pm, c->pl, dR,
                        ldR);
     DG_HIP(c, hipGetLastError());
+    return DG_OK;
+}
+
+int dg_ctemp_moments_fp64(dg_ctx *c, int k0, int KB, int nc, double *dPart, double *dS) {
+    const double it = c->last_isDA ? 1.0 / c->last_temp : 1.0;
+    const int nt = (c->D + CM - 1) / CM;
+    const long tiles = (c->N + CM - 1) / CM;
+    const long chunk = ((tiles + nc - 1) / nc) * CM;
+    dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)KB, (unsigned)nc);
+    hipLaunchKernelGGL(k_ctemp_moments, grid, dim3(NT), 0, c->stream, c->T64, c->tt, c->N,
+                       c->Y64, c->yy, k0, c->D, c->ldt, -c->beta / 2.0, it, 1.0 / c->K, c->pm,
+                       c->pl, chunk, dPart);
+    DG_HIP(c, hipGetLastError());
+    if (nc > 1) {
+        const long len = (long)KB * 2 * c->D * c->D;
+        hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
+                           c->stream, dPart, nc, len, dS);
+        DG_HIP(c, hipGetLastError());
+    }
     return DG_OK;
 }
 

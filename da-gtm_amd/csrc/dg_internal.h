@@ -43,6 +43,7 @@ struct dg_ctx {
     double *A0 = nullptr;          // M1 x M1 unfactored copy
     double *Bm = nullptr;          // M1 x D
     double *G = nullptr;           // K
+    double *Gm = nullptr;          // K   G used by the last M-step (zero before one)
     double *RT = nullptr;          // K x D
     double *pm = nullptr;          // N   per-point max exponent x_max (pass 1)
     double *pl = nullptr;          // N   per-point log S2 (pass 1)
@@ -100,6 +101,12 @@ int dg_hip_check(dg_ctx *ctx, hipError_t e, const char *what);
 int dg_estep_fp64(dg_ctx *ctx, int isDA, double temp);
 int dg_resp_dense_fp64(dg_ctx *ctx, double *dR, long ldR);
 int dg_map_fp64(dg_ctx *ctx, const double *dX, int L, double *dM);
+// S1, S2 of centres k0 .. k0+KB-1 into dS (KB x 2 x D x D, lower triangle
+// valid) from nc N-chunk partials in dPart (dS == dPart when nc == 1).
+int dg_ctemp_moments_fp64(dg_ctx *ctx, int k0, int KB, int nc, double *dPart, double *dS);
+
+// ---- host eigenvalues (dg_eig.cpp) ----
+extern "C" int dg_sym_eigvals(int n, const double *A, long lda, double *w);
 
 // ---- mixed kernels (dg_mixed.hip) ----
 int dg_mixed_prepare_t(dg_ctx *ctx);

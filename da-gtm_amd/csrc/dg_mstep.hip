@@ -225,21 +225,18 @@ __global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
 constexpr int kLdsN = 126;   // largest M+1 whose factor fits in LDS
 static_assert(kLdsN <= 128, "the Cholesky and solve kernels give each thread at most 4 rows");
 
-// Cholesky of A0 + (ridge + jitter) I by one workgroup (n <= 126), same
-// pivots and retries as k_chol_solve.  Thread (tx, ty) of 32 x 32 keeps the
-// lower-triangle elements A[i][k], i = ty + 32 a, k = tx + 32 b, in registers
-// for the whole factorization.  Step j reads the still unscaled pivot column
-// j from an LDS ping-pong buffer, writes column j of L, updates its elements
-// (A[i][k] -= A[i][j] A[k][j] / A[j][j]) and the owners of column j + 1 publish
-// it in the other buffer: one barrier per step, no LDS read-modify-write.
-// Writes the lower triangle of L.
+// Cholesky of A0 + (ridge + jitter) I with the matrix resident in LDS (one
+// workgroup, n <= 126); same pivots and retries as k_chol_solve.  One barrier
+// per column: step j updates the trailing lower triangle with the column
+// still unscaled, A[i][k] -= A[i][j] A[k][j] / p_j, and scales column j - 1,
+// which no step reads any more.  Writes the factor's lower triangle to L.
 __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
                                                    double *__restrict__ L, int n, double ridge,
                                                    double *__restrict__ scal) {
-    __shared__ double col[2][128];
+    __shared__ double As[kLdsN * kLdsN];
     __shared__ double tr;
-    const int tid = threadIdx.x;
-    const int tx = tid & 31, ty = tid >> 5;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tx = tid & 31, ty = tid >> 5;          // 32 x 32 threads over (row, column)
     if (tid == 0) {
         double s = 0.0;
         for (int i = 0; i < n; ++i) s += A0[(long)i * n + i];
@@ -251,62 +248,35 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
     for (int attempt = 0; attempt < 9 && !ok; ++attempt) {
         if (attempt == 1) jit = 1e-13 * tr / n;
         else if (attempt > 1) jit *= 10.0;
-        double v[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int i = ty + 32 * a, k = tx + 32 * b;
-                v[a][b] = (i < n && k <= i) ? A0[(long)i * n + k] + ((i == k) ? ridge + jit : 0.0)
-                                            : 0.0;
-            }
-        // publish column 0
-        if (tx == 0) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int i = ty + 32 * a;
-                if (i < n) col[0][i] = v[a][0];
-            }
+        for (int e = tid; e < n * n; e += nt) {
+            const int i = e / n, j = e % n;
+            As[e] = A0[e] + ((i == j) ? ridge + jit : 0.0);
         }
+        __syncthreads();
         bool failed = false;
+        double pprev = 0.0;
         for (int j = 0; j < n; ++j) {
-            __syncthreads();
-            const double *cj = col[j & 1];
-            const double p = cj[j];
-            if (!(p > 0.0)) { failed = true; break; }   // the same for every thread
+            const double p = As[j * n + j];           // unscaled pivot
+            if (!(p > 0.0)) { failed = true; break; } // the same for every thread
+            if (j > 0) {                              // scale column j - 1
+                const double sq = sqrt(pprev);
+                const double inv = 1.0 / sq;
+                const int i = j - 1 + tid;
+                if (i == j - 1) As[(j - 1) * n + (j - 1)] = sq;
+                else if (i < n) As[i * n + (j - 1)] *= inv;
+            }
             const double rp = 1.0 / p;
-            if (tx == (j & 31)) {                        // column j of L
-                const double sq = sqrt(p), inv = 1.0 / sq;
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = ty + 32 * a;
-                    if (i == j) L[(long)j * n + j] = sq;
-                    else if (i > j && i < n) L[(long)i * n + j] = cj[i] * inv;
-                }
+            for (int i = j + 1 + ty; i < n; i += 32) {
+                const double aij = As[i * n + j] * rp;
+                for (int k = j + 1 + tx; k <= i; k += 32)
+                    As[i * n + k] = fma(-aij, As[k * n + j], As[i * n + k]);
             }
-            double ci[4], ck[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int i = ty + 32 * a, k = tx + 32 * a;
-                ci[a] = (i > j && i < n) ? cj[i] * rp : 0.0;
-                ck[a] = (k > j && k < n) ? cj[k] : 0.0;
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) v[a][b] = fma(-ci[a], ck[b], v[a][b]);
-            // owners of column j + 1 publish it (unscaled) for the next step
-            if (j + 1 < n && tx == ((j + 1) & 31)) {
-                const int bj = (j + 1) >> 5;
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = ty + 32 * a;
-                    double val = v[a][0];
-#pragma unroll
-                    for (int b = 1; b < 4; ++b) val = (bj == b) ? v[a][b] : val;
-                    if (i > j && i < n) col[(j + 1) & 1][i] = val;
-                }
-            }
+            pprev = p;
+            __syncthreads();
+        }
+        if (!failed) {                                // scale the last column
+            if (tid == 0) As[(n - 1) * n + (n - 1)] = sqrt(pprev);
+            __syncthreads();
         }
         ok = !failed;
         __syncthreads();
@@ -315,6 +285,7 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
         scal[kScalJitter] = jit;
         scal[kScalSolveOK] = ok ? 1.0 : 0.0;
     }
+    for (int e = tid; e < n * n; e += nt) L[e] = As[e];
 }
 
 // W[:, c0:c0+32] from L y = b, L^T x = y (n <= 126): L in LDS, thread

@@ -16,10 +16,10 @@ LIB_PATH = os.path.join(HERE, "libdagtm_gpu.so")
 
 DG_OK, DG_EINVAL, DG_ENOMEM, DG_EHIP, DG_ESTATE, DG_ESOLVE = 0, 1, 2, 3, 4, 5
 FP64, MIXED = 0, 1
-SCHED_EXP, SCHED_LINEAR = 0, 1
+SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
-           "dg_set_state", "dg_estep", "dg_mstep", "dg_anneal", "dg_map",
+           "dg_set_state", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
            "dg_get_timing"]
 
@@ -78,6 +78,7 @@ def load():
     L.dg_mstep.argtypes = [vp, C.POINTER(C.c_double)]
     L.dg_anneal.argtypes = [vp, C.POINTER(DgAnnealParams), C.POINTER(DgAnnealResult),
                             ITER_CB, vp]
+    L.dg_next_ctemp.argtypes = [vp, C.c_double, C.c_double, vp, C.POINTER(C.c_double), vp]
     L.dg_map.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
     L.dg_resp_dense.argtypes = [vp, _dp, C.c_long]
     L.dg_download.argtypes = [vp, vp, C.c_long, vp, C.c_long, C.POINTER(C.c_double)]
@@ -161,6 +162,19 @@ class Engine:
         cb = ITER_CB(_cb)
         self._chk(self.L.dg_anneal(self.h, C.byref(p), C.byref(r), cb, None))
         return r, rows
+
+    def next_ctemp(self, temp, eps, gg=None):
+        """Critical temperatures at the current state (needs an estep there).
+        gg=None uses the G of the last mstep, as the reference's loop does.
+        Returns (max, per-centre array)."""
+        g = None if gg is None else _f64(gg)
+        if g is not None and g.shape != (self.K,):
+            raise ValueError("gg must have K entries")
+        ck = np.zeros(self.K); cmax = C.c_double()
+        self._chk(self.L.dg_next_ctemp(self.h, float(temp), float(eps),
+                                       None if g is None else g.ctypes.data, C.byref(cmax),
+                                       ck.ctypes.data))
+        return cmax.value, ck
 
     def map(self, X):
         X = _f64(X)

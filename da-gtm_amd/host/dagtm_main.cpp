@@ -34,7 +34,8 @@ struct Opts {
     int L = 3, K = 125, M = 8, maxloop = 0, ckp_every = 100, verbose = 0, device = 0;
     double alpha = 0.99, delta = 0.1, eps = 1e-6, start_temp = 1.0, lambda = 0.0, s = 2.0;
     bool isDA = true, need_ctemp = true, checkpointing = true, log_progress = false;
-    bool save_R = false, linear = false, mixed = false, init_only = false, interp = false;
+    bool save_R = false, mixed = false, init_only = false, interp = false;
+    int schedule = DG_SCHED_EXP;
     unsigned long seed = 0;
     std::string in = "INPUT.h5", out, restart, ckp_prefix = "ckp.";
     std::string dT = "T", dX = "X", dFI = "FI", dW = "W", dY = "Y", dR = "R", dbeta = "beta",
@@ -48,7 +49,7 @@ void usage(const char *prog) {
                  "-a number    : alpha (default: 0.99)\n"
                  "-b name      : dataset name for beta (default: beta)\n"
                  "-B name      : dataset name for label (default: lb)\n"
-                 "-c schedule  : cooling schedule [exp|linear] (default: exp)\n"
+                 "-c schedule  : cooling schedule [auto|exp|linear] (default: exp)\n"
                  "-e number    : precision (convergence threshold, default 1e-6)\n"
                  "-f dataset   : dataset name for FI (default: FI)\n"
                  "-h           : show help\n"
@@ -97,12 +98,10 @@ int parse(int argc, char **argv, Opts &o) {
             case 'b': o.dbeta = optarg; break;
             case 'B': o.dlb = optarg; break;
             case 'c':
-                if (!strcasecmp(optarg, "exp")) o.linear = false;
-                else if (!strcasecmp(optarg, "linear")) o.linear = true;
-                else if (!strcasecmp(optarg, "auto")) {
-                    std::fprintf(stderr, "Cooling schedule 'auto' is not supported by this engine\n");
-                    return 1;
-                } else {
+                if (!strcasecmp(optarg, "exp")) o.schedule = DG_SCHED_EXP;
+                else if (!strcasecmp(optarg, "linear")) o.schedule = DG_SCHED_LINEAR;
+                else if (!strcasecmp(optarg, "auto")) o.schedule = DG_SCHED_AUTO;
+                else {
                     std::fprintf(stderr, "Unknown schedule option : %s\n", optarg);
                     return 1;
                 }
@@ -298,7 +297,7 @@ int main(int argc, char **argv) {
         return ok ? 0 : 1;
     }
 
-    // ---- annealing loop (dagtm.cxx:1508-2206, EXP / LINEAR / EM)
+    // ---- annealing loop (dagtm.cxx:1508-2206, AUTO / EXP / LINEAR / EM)
     const int QN = 50;
     std::vector<double> Q(QN);
     std::vector<double> hT, hL, hF;
@@ -349,8 +348,16 @@ int main(int argc, char **argv) {
             if (o.maxloop && nloop >= o.maxloop) { isLast = true; break; }
         }
         if (o.isDA && !isLast) {
-            if (o.linear) temp -= o.delta;
-            else temp *= o.alpha;
+            if (o.schedule == DG_SCHED_LINEAR) {
+                temp -= o.delta;
+            } else if (o.schedule == DG_SCHED_AUTO) {
+                // dagtm.cxx:1919-1935
+                double ct = 0.0;
+                CHECK_DG(dg_next_ctemp(ctx, temp, o.eps, nullptr, &ct, nullptr));
+                temp = (ct >= temp) ? temp * o.alpha : ct;
+            } else {
+                temp *= o.alpha;
+            }
         } else {
             break;
         }

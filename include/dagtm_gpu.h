@@ -83,11 +83,12 @@ int dg_estep(dg_ctx *ctx, int isDA, double temp, dg_estep_out *out);
  * dagtm_mpi_beta (mpicomm.c:276-302).  *beta_out may be NULL. */
 int dg_mstep(dg_ctx *ctx, double *beta_out);
 
-enum { DG_SCHED_EXP = 0, DG_SCHED_LINEAR = 1 };
+enum { DG_SCHED_EXP = 0, DG_SCHED_LINEAR = 1, DG_SCHED_AUTO = 2 };
 
 typedef struct {
     int    isDA;        /* 1 = deterministic annealing, 0 = plain EM (-m) */
-    int    schedule;    /* DG_SCHED_EXP (-c exp) or DG_SCHED_LINEAR (-c linear) */
+    int    schedule;    /* DG_SCHED_EXP (-c exp), DG_SCHED_LINEAR (-c linear) or
+                           DG_SCHED_AUTO (-c auto) */
     int    maxloop;     /* -j, 0 = unlimited */
     double alpha;       /* -a, EXP cooling factor */
     double delta;       /* LINEAR step (dagtm.cxx:2169-2171) */
@@ -107,11 +108,30 @@ typedef void (*dg_iter_cb)(void *user, int nloop, int lloop, double temp,
                            double nfree, double llh, double diff,
                            double beta);
 
-/* The annealing loop dagtm.cxx:1508-2206 for the EXP and LINEAR schedules
- * and EM mode, with the reference's convergence tests (|diff| < eps, the
- * 50-iteration moving average, qual_old = 0 at start). */
+/* The annealing loop dagtm.cxx:1508-2206 for the EXP, LINEAR and AUTO
+ * schedules and EM mode, with the reference's convergence tests (|diff| <
+ * eps, the 50-iteration moving average, qual_old = 0 at start).  AUTO moves
+ * to dg_next_ctemp's temperature, or multiplies by alpha when that is not
+ * below the current one (dagtm.cxx:1919-1935). */
 int dg_anneal(dg_ctx *ctx, const dg_anneal_params *p, dg_anneal_result *res,
               dg_iter_cb cb, void *user);
+
+/* Critical temperatures of the current state, for AUTO cooling.  For each
+ * centre k, with S1 = sum_n R_kn x x^T and S2 = sum_n R_kn^2 x x^T over
+ * x = t_n - y_k, the eigenvalues of H = [[2S1 - S2, -S2], [-S2, 2S1 - S2]]
+ * at or above limit = (temp - eps) g_k 2 / beta count as 0, and
+ * ctemp_k = beta / (2 g_k) * max of them (HUGE_VAL when g_k = 0);
+ * *ctemp_max = max_k ctemp_k.  Replaces dagtm_mpi_next_ctemp and its helper
+ * (mpicomm.c:561-686) with dagtm_vector_max (comm.c:2149-2161).
+ *   gg        K weights g_k, or NULL for the G of the last dg_mstep (zero
+ *             before the first), which is what the reference passes
+ *             (vggsub, dagtm.cxx:1715 and 1922)
+ *   ctemp_k   K outputs, may be NULL
+ * R is the responsibility matrix of the last dg_estep (which must be at the
+ * current state), computed in FP64 in both precisions.  Cost O(K N D^2) on
+ * the device plus 2K symmetric D x D eigenvalue problems on the host. */
+int dg_next_ctemp(dg_ctx *ctx, double temp, double eps, const double *gg,
+                  double *ctemp_max, double *ctemp_k);
 
 /* MAP projection M = R^T X (N x L, ldM >= L) with the responsibilities at
  * the current model state and the last E-step's temperature (an E-step is

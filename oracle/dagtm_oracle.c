@@ -575,7 +575,7 @@ int orc_model_alloc(orc_model *m)
     m->R = (double *)malloc(sizeof(double) * KN);
     m->g = (double *)malloc(sizeof(double) * (size_t)m->N);
     m->h = (double *)malloc(sizeof(double) * (size_t)m->N);
-    m->G = (double *)malloc(sizeof(double) * (size_t)m->K);
+    m->G = (double *)calloc((size_t)m->K, sizeof(double));   /* zero before the first M-step */
     m->A = (double *)malloc(sizeof(double) * (size_t)m->M1 * m->M1);
     m->B = (double *)malloc(sizeof(double) * (size_t)m->M1 * m->D);
     m->FIR = (double *)malloc(sizeof(double) * (size_t)m->M1 * (size_t)m->N);
@@ -690,8 +690,19 @@ int orc_anneal(orc_model *m, const orc_anneal_params *p,
             if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
         }
         if (p->isDA && !isLast) {
-            if (p->schedule == ORC_SCHED_LINEAR) temp -= p->delta;
-            else temp *= p->alpha;
+            if (p->schedule == ORC_SCHED_LINEAR) {
+                temp -= p->delta;
+            } else if (p->schedule == ORC_SCHED_AUTO) {
+                /* dagtm.cxx:1918-1934; R is the last E-step's, G the last
+                 * M-step's (the reference's vggsub is refreshed only after the
+                 * convergence test, dagtm.cxx:1715) */
+                double ct = orc_next_ctemp(m->T, m->N, m->D, m->Y, m->K, m->R, m->G, temp,
+                                           m->beta, p->eps, NULL);
+                if (ct >= temp) ct = temp * p->alpha;
+                temp = ct;
+            } else {
+                temp *= p->alpha;
+            }
         } else {
             break;
         }
@@ -700,6 +711,70 @@ int orc_anneal(orc_model *m, const orc_anneal_params *p,
     res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual;
     res->beta = m->beta;
     return 0;
+}
+
+double orc_next_ctemp(const double *T, long N, int D, const double *Y, int K,
+                      const double *R, const double *gg, double temp, double beta,
+                      double eps, double *ctemp_k)
+{
+    const double GG_MIN = 1e-250;                 /* mpicomm.c:617-628 */
+    const int n2 = 2 * D;
+    double *H = (double *)malloc(sizeof(double) * (size_t)n2 * n2);
+    double *lam = (double *)malloc(sizeof(double) * (size_t)n2);
+    double *S1 = (double *)malloc(sizeof(double) * (size_t)D * D);
+    double *S2 = (double *)malloc(sizeof(double) * (size_t)D * D);
+    double *xs = (double *)malloc(sizeof(double) * (size_t)D);
+    double *x2 = (double *)malloc(sizeof(double) * (size_t)D);
+    double *V = (double *)malloc(sizeof(double) * (size_t)n2 * n2);
+    double cmax = -HUGE_VAL;
+    for (int k = 0; k < K; k++) {
+        double ck;
+        const double g = gg[k];
+        if (g == 0.0) {
+            ck = HUGE_VAL;
+        } else {
+            memset(S1, 0, sizeof(double) * (size_t)D * D);
+            memset(S2, 0, sizeof(double) * (size_t)D * D);
+            for (long n = 0; n < N; n++) {
+                const double r = R[(size_t)k * N + n];
+                const double sr = sqrt(r);
+                for (int a = 0; a < D; a++) {
+                    xs[a] = (T[IDX(n, a, D)] - Y[IDX(k, a, D)]) * sr;   /* sqrt(r) x */
+                    x2[a] = xs[a] * sr;                                    /* r x */
+                }
+                for (int a = 0; a < D; a++)
+                    for (int b = 0; b < D; b++) {
+                        S1[IDX(a, b, D)] += 2.0 * xs[a] * xs[b];
+                        S2[IDX(a, b, D)] += x2[a] * x2[b];
+                    }
+            }
+            for (int a = 0; a < D; a++)
+                for (int b = 0; b < D; b++) {
+                    const double s11 = S1[IDX(a, b, D)] - S2[IDX(a, b, D)];
+                    const double s12 = -S2[IDX(a, b, D)];
+                    H[IDX(a, b, n2)] = s11;
+                    H[IDX(a + D, b + D, n2)] = s11;
+                    H[IDX(a, b + D, n2)] = s12;
+                    H[IDX(a + D, b, n2)] = s12;
+                }
+            if (g < GG_MIN)
+                for (long e = 0; e < (long)n2 * n2; e++) H[e] /= GG_MIN;
+            orc_eig_sym(n2, H, lam, V);
+            if (g < GG_MIN)
+                for (int i = 0; i < n2; i++) lam[i] *= GG_MIN;
+            const double limit = (temp - eps) * g * 2.0 / beta;
+            double lmax = -HUGE_VAL;
+            for (int i = 0; i < n2; i++) {
+                const double l = (lam[i] >= limit) ? 0.0 : lam[i];
+                if (l > lmax) lmax = l;
+            }
+            ck = beta / 2.0 / g * lmax;
+        }
+        if (ctemp_k) ctemp_k[k] = ck;
+        if (ck > cmax) cmax = ck;
+    }
+    free(H); free(lam); free(S1); free(S2); free(xs); free(x2); free(V);
+    return cmax;
 }
 
 /* dagtm.cxx:2347-2355: M = R^T X (N x L) with the R left by the last

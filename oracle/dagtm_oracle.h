@@ -77,7 +77,7 @@ double orc_beta(const double *Y, const double *T, double *R, long N, int K,
 
 /* ---- one materialised E+M iteration and the annealing loop ----------- */
 enum { ORC_SOLVE_PINV = 0, ORC_SOLVE_CHOL = 1 };
-enum { ORC_SCHED_EXP = 0, ORC_SCHED_LINEAR = 1 };
+enum { ORC_SCHED_EXP = 0, ORC_SCHED_LINEAR = 1, ORC_SCHED_AUTO = 2 };
 
 typedef struct {
     long N; int D, K, M1;
@@ -128,6 +128,16 @@ typedef void (*orc_iter_cb)(void *user, int nloop, int lloop, double temp,
 
 int orc_anneal(orc_model *m, const orc_anneal_params *p,
                orc_anneal_result *res, orc_iter_cb cb, void *user);
+
+/* Next critical temperature below temp (dagtm_mpi_next_ctemp and its helper,
+ * mpicomm.c:561-675): per centre k, x_n = t_n - y_k, S1 = sum_n R_kn x x^T,
+ * S2 = sum_n R_kn^2 x x^T, H = [[2 S1 - S2, -S2], [-S2, 2 S1 - S2]] (2D x 2D);
+ * eigenvalues of H at or above limit = (temp - eps) gg_k 2 / beta count as 0
+ * (dagtm_vector_max, comm.c:2149-2161) and ctemp_k = beta / (2 gg_k) times the
+ * largest; HUGE_VAL when gg_k = 0.  Returns max_k ctemp_k; ctemp_k may be NULL. */
+double orc_next_ctemp(const double *T, long N, int D, const double *Y, int K,
+                      const double *R, const double *gg, double temp, double beta,
+                      double eps, double *ctemp_k);
 
 /* MAP projection M = R^T X (dagtm.cxx:2347-2355): N x L */
 void orc_map(const orc_model *m, const double *X, int L, double *Mout);

@@ -45,7 +45,7 @@ ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double
                       C.c_double, C.c_double, C.c_double)
 
 SOLVE_PINV, SOLVE_CHOL = 0, 1
-SCHED_EXP, SCHED_LINEAR = 0, 1
+SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 
 
 def lib():
@@ -93,6 +93,9 @@ def lib():
         L.orc_anneal.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcAnnealParams),
                                  C.POINTER(OrcAnnealResult), ITER_CB, C.c_void_p]
         L.orc_map.argtypes = [C.POINTER(OrcModel), _dp, C.c_int, _dp]
+        L.orc_next_ctemp.argtypes = [_dp, C.c_long, C.c_int, _dp, C.c_int, _dp, _dp, C.c_double,
+                                     C.c_double, C.c_double, _dp]
+        L.orc_next_ctemp.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -290,6 +293,16 @@ class Model:
         cb = ITER_CB(_cb)
         lib().orc_anneal(C.byref(self.m), C.byref(p), C.byref(r), cb, None)
         return r, rows
+
+    def next_ctemp(self, temp, eps, gg=None):
+        """Next critical temperature below temp from the last E-step's R and
+        (by default) the last M-step's G, as the reference's loop passes them.
+        Returns (max, per-centre array)."""
+        g = self.G if gg is None else _f64(gg)
+        ck = np.zeros(self.m.K)
+        c = lib().orc_next_ctemp(self.T, self.m.N, self.m.D, self.Y, self.m.K, self.R, g,
+                                 float(temp), float(self.m.beta), float(eps), ck)
+        return c, ck
 
     def map(self, X):
         X = _f64(X)

@@ -49,5 +49,6 @@ def test_null_context_is_rejected():
     lib = dg.load()
     assert lib.dg_estep(None, 1, 1.0, None) == dg.DG_EINVAL
     assert lib.dg_mstep(None, None) == dg.DG_EINVAL
+    assert lib.dg_next_ctemp(None, 1.0, 0.0, None, None, None) == dg.DG_EINVAL
     assert lib.dg_destroy(None) == dg.DG_OK
     assert lib.dg_last_error(None) == b"null context"

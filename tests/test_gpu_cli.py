@@ -85,6 +85,31 @@ def test_cli_run_matches_oracle(tmp_path, oil_state, oil_input):
     assert np.all(np.abs(Mo) <= 1.0 + 1e-12)                  # convex combinations of the grid
 
 
+def test_cli_auto_schedule_matches_oracle(tmp_path, oil_state, oil_input):
+    _say("case: dagtm -c auto on the Oil data, K=125 M=8, 40 iterations, fp64")
+    init = str(tmp_path / "init.h5")
+    common = ["-i", oil_input, "-K", "125", "-M", "8", "-L", "3", "-s", "23", "-a", "0.9"]
+    subprocess.run([DAGTM, "--init-only", "-o", init] + common, check=True)
+    r = subprocess.run([DAGTM, "-n", "-j", "40", "-c", "auto"] + common, check=True,
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    rows = _rows(r.stdout)
+    T = oil_state["T"]
+    FI = h5util.read(init, "FI").reshape(125, 9)
+    Y0 = h5util.read(init, "Y").reshape(125, 12)
+    b0 = h5util.read(init, "beta")[0]
+    t0 = h5util.read(init, "temp")[0]
+    kw = dict(isDA=True, schedule=O.SCHED_AUTO, alpha=0.9, eps=1e-6, start_temp=t0, maxloop=40)
+    _, orows = O.Model(T, FI, Y0, b0, solver=O.SOLVE_CHOL).anneal(**kw)
+    _, prows = O.Model(T, FI, Y0 * (1.0 + 1e-13), b0, solver=O.SOLVE_CHOL).anneal(**kw)
+    assert len(orows) == len(rows) == len(prows) == 40
+    assert len(set(o[2] for o in orows)) > 2               # AUTO moved the temperature
+    for g, o, p in zip(rows, orows, prows):
+        assert g[0] == o[0] and g[1] == o[1]
+        assert abs(g[2] - o[2]) <= 1e-7 + 1e-7 * o[2] + 50.0 * abs(p[2] - o[2]), (g[0], g[2], o[2])
+        tol = 2e-7 + 1e-7 * abs(o[4]) + 50.0 * abs(p[4] - o[4])
+        assert abs(g[4] - o[4]) <= tol, (g[0], g[4], o[4], tol)
+
+
 def test_checkpoint_restart_reproduces_trajectory(tmp_path, oil_input):
     _say("case: dagtm checkpoint every 10, restart from iteration 20")
     common = ["-i", oil_input, "-K", "125", "-M", "8", "-L", "3", "-s", "5", "-e", "0",
