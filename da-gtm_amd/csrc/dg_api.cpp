@@ -50,14 +50,53 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
-void free_problem(dg_ctx *c) {
-    dfree(c->T64); dfree(c->tt); dfree(c->FI); dfree(c->Y64); dfree(c->yy);
-    dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm); dfree(c->G); dfree(c->Gm); dfree(c->RT);
-    dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
+// Device state whose size depends on K: basis, centres, per-centre
+// statistics and the partial sums of their reductions.
+void free_kstate(dg_ctx *c) {
+    dfree(c->FI); dfree(c->Y64); dfree(c->yy); dfree(c->G); dfree(c->Gm); dfree(c->RT);
     dfree(c->partRT); dfree(c->partG); dfree(c->partAB);
+}
+
+void free_problem(dg_ctx *c) {
+    free_kstate(c);
+    dfree(c->T64); dfree(c->tt); dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm);
+    dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
     dg_mixed_free(c);
     c->loaded = false;
     c->have_estep = false;
+    c->have_w = false;
+}
+
+// Allocates the K-dependent state for c->K (c->N, c->D, c->M1 set); Y64 and
+// Gm start at zero.
+int alloc_kstate(dg_ctx *c) {
+    const long N = c->N, K = c->K;
+    const int D = c->D, M1 = c->M1;
+    const size_t ld = (size_t)c->ldt;
+    // N chunks of E-pass 2: enough (centre tile x chunk) workgroups to fill
+    // 256 CUs several times, each chunk at least one 64-point tile.
+    const long tiles = (N + 63) / 64;
+    const long ktiles = (K + 63) / 64;
+    long nc = (1024 + ktiles - 1) / ktiles;
+    if (nc > tiles) nc = tiles;
+    if (nc < 1) nc = 1;
+    c->NC = (int)nc;
+    // A/B reduction: at most 64 chunks of centres, at least 64 centres each
+    long nch = (K + 63) / 64;
+    if (nch > 64) nch = 64;
+    c->nchAB = (int)(nch < 1 ? 1 : nch);
+    int rc;
+    if ((rc = dalloc(c, &c->FI, (size_t)K * M1)) || (rc = dalloc(c, &c->Y64, (size_t)K * ld)) ||
+        (rc = dalloc(c, &c->yy, K)) || (rc = dalloc(c, &c->G, K)) || (rc = dalloc(c, &c->Gm, K)) ||
+        (rc = dalloc(c, &c->RT, (size_t)K * D)) ||
+        (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
+        (rc = dalloc(c, &c->partG, (size_t)c->NC * K)) ||
+        (rc = dalloc(c, &c->partAB, std::max((size_t)c->nchAB * ((size_t)M1 * M1 + (size_t)M1 * D),
+                                             (size_t)(K + 63) / 64))))
+        return rc;
+    DG_HIP(c, hipMemsetAsync(c->Y64, 0, (size_t)K * ld * sizeof(double), c->stream));
+    DG_HIP(c, hipMemsetAsync(c->Gm, 0, (size_t)K * sizeof(double), c->stream));
+    return DG_OK;
 }
 
 int sync_scalars(dg_ctx *c) {
@@ -155,37 +194,16 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     c->N = N; c->D = D; c->K = K; c->M1 = M1; c->beta = beta;
     c->ldt = ((D + 15) / 16) * 16;
     const size_t ld = (size_t)c->ldt;
-    // N chunks of E-pass 2: enough (centre tile x chunk) workgroups to fill
-    // 256 CUs several times, each chunk at least one 64-point tile.
-    const long tiles = (N + 63) / 64;
-    const long ktiles = (K + 63) / 64;
-    long nc = (1024 + ktiles - 1) / ktiles;
-    if (nc > tiles) nc = tiles;
-    if (nc < 1) nc = 1;
-    c->NC = (int)nc;
-    // A/B reduction: at most 64 chunks of centres, at least 64 centres each
-    long nch = (K + 63) / 64;
-    if (nch > 64) nch = 64;
-    c->nchAB = (int)(nch < 1 ? 1 : nch);
-
     if ((rc = dalloc(c, &c->T64, (size_t)N * ld)) || (rc = dalloc(c, &c->tt, N)) ||
-        (rc = dalloc(c, &c->FI, (size_t)K * M1)) || (rc = dalloc(c, &c->Y64, (size_t)K * ld)) ||
-        (rc = dalloc(c, &c->yy, K)) || (rc = dalloc(c, &c->W, (size_t)M1 * D)) ||
-        (rc = dalloc(c, &c->A, (size_t)M1 * M1)) || (rc = dalloc(c, &c->A0, (size_t)M1 * M1)) ||
-        (rc = dalloc(c, &c->Bm, (size_t)M1 * D)) || (rc = dalloc(c, &c->G, K)) ||
-        (rc = dalloc(c, &c->Gm, K)) ||
-        (rc = dalloc(c, &c->RT, (size_t)K * D)) || (rc = dalloc(c, &c->pm, N)) ||
-        (rc = dalloc(c, &c->pl, N)) || (rc = dalloc(c, &c->logg, N)) ||
-        (rc = dalloc(c, &c->logh, N)) || (rc = dalloc(c, &c->partRT, (size_t)c->NC * K * D)) ||
-        (rc = dalloc(c, &c->partG, (size_t)c->NC * K)) ||
-        (rc = dalloc(c, &c->partAB, std::max((size_t)c->nchAB * ((size_t)M1 * M1 + (size_t)M1 * D),
-                                             (size_t)(K + 63) / 64)))) {
+        (rc = dalloc(c, &c->W, (size_t)M1 * D)) || (rc = dalloc(c, &c->A, (size_t)M1 * M1)) ||
+        (rc = dalloc(c, &c->A0, (size_t)M1 * M1)) || (rc = dalloc(c, &c->Bm, (size_t)M1 * D)) ||
+        (rc = dalloc(c, &c->pm, N)) || (rc = dalloc(c, &c->pl, N)) ||
+        (rc = dalloc(c, &c->logg, N)) || (rc = dalloc(c, &c->logh, N)) ||
+        (rc = alloc_kstate(c))) {
         free_problem(c);
         return rc;
     }
     DG_HIP(c, hipMemsetAsync(c->T64, 0, (size_t)N * ld * sizeof(double), c->stream));
-    DG_HIP(c, hipMemsetAsync(c->Y64, 0, (size_t)K * ld * sizeof(double), c->stream));
-    DG_HIP(c, hipMemsetAsync(c->Gm, 0, (size_t)K * sizeof(double), c->stream));
     DG_HIP(c, hipMemcpy2DAsync(c->T64, ld * sizeof(double), T, (size_t)ldT * sizeof(double),
                                (size_t)D * sizeof(double), (size_t)N, hipMemcpyHostToDevice,
                                c->stream));
@@ -220,6 +238,7 @@ int dg_set_state(dg_ctx *c, const double *Y, long ldY, double beta) {
     if ((rc = dg_rownorms(c, c->Y64, c->K, c->ldt, c->D, c->yy))) return rc;
     DG_HIP(c, hipMemsetAsync(c->Gm, 0, (size_t)c->K * sizeof(double), c->stream));
     c->beta = beta;
+    c->have_w = false;
     c->have_estep = false;
     if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
     DG_HIP(c, hipStreamSynchronize(c->stream));
@@ -277,6 +296,7 @@ int dg_mstep(dg_ctx *c, double *beta_out) {
     c->have_estep = false;
     if (c->h_scal[kScalSolveOK] != 1.0)
         return dg_fail(c, DG_ESOLVE, "M-step Cholesky failed after jitter retries");
+    c->have_w = true;
     const double s = c->sum_tt + c->h_scal[kScalBetaSum];
     c->beta = ((double)c->N * c->D) / s;
     if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
@@ -343,6 +363,38 @@ int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_it
     } while (!isLast);
     res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
     res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual; res->beta = c->beta;
+    return DG_OK;
+}
+
+int dg_set_basis(dg_ctx *c, const double *FI, int K, int M1, long ldFI, const double *W, long ldW) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_set_basis before dg_upload");
+    if (!FI || K < 1 || M1 != c->M1 || ldFI < M1 || (W && ldW < c->D))
+        return dg_fail(c, DG_EINVAL, "dg_set_basis: bad shape or stride (M+1 must not change)");
+    if (!W && !c->have_w)
+        return dg_fail(c, DG_ESTATE, "dg_set_basis: no M-step since the last upload; pass W");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    free_kstate(c);
+    c->K = K;
+    c->have_estep = false;
+    if ((rc = alloc_kstate(c))) { free_problem(c); return rc; }
+    DG_HIP(c, hipMemcpy2DAsync(c->FI, (size_t)M1 * sizeof(double), FI, (size_t)ldFI * sizeof(double),
+                               (size_t)M1 * sizeof(double), (size_t)K, hipMemcpyHostToDevice,
+                               c->stream));
+    if (W)
+        DG_HIP(c, hipMemcpy2DAsync(c->W, (size_t)c->D * sizeof(double), W,
+                                   (size_t)ldW * sizeof(double), (size_t)c->D * sizeof(double),
+                                   (size_t)M1, hipMemcpyHostToDevice, c->stream));
+    c->have_w = true;
+    if ((rc = dg_centres_from_w(c))) return rc;
+    // the mixed path's operand layout and chunking depend on K: rebuild it
+    // from the resident FP64 data
+    if (c->cfg.precision == DG_MIXED &&
+        ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))))
+        return rc;
+    DG_HIP(c, hipStreamSynchronize(c->stream));
     return DG_OK;
 }
 

@@ -26,6 +26,7 @@ struct dg_ctx {
     int D = 0, K = 0, M1 = 0;
     bool loaded = false;
     bool have_estep = false;       // G/RT/pm/pl valid for the current Y
+    bool have_w = false;           // W holds the last M-step's (or a caller's) weights
     bool last_isDA = true;
     double last_temp = 1.0;
     double beta = 0.0;
@@ -116,6 +117,8 @@ void dg_mixed_free(dg_ctx *ctx);
 
 // ---- M-step (dg_mstep.hip) ----
 int dg_mstep_run(dg_ctx *ctx);
+// Y = FI W and |y_k|^2 from the context's FI and W.
+int dg_centres_from_w(dg_ctx *ctx);
 int dg_rownorms(dg_ctx *ctx, const double *X, long rows, int ld, int D, double *out);
 int dg_reduce_logs(dg_ctx *ctx);
 int dg_sum_into(dg_ctx *ctx, const double *v, long n, int slot);

@@ -474,6 +474,14 @@ int dg_sum_into(dg_ctx *c, const double *v, long n, int slot) {
     return DG_OK;
 }
 
+int dg_centres_from_w(dg_ctx *c) {
+    const int D = c->D, K = c->K;
+    hipLaunchKernelGGL(k_phiw, dim3((unsigned)((K + GT - 1) / GT), (unsigned)((D + GT - 1) / GT)),
+                       dim3(256), 0, c->stream, c->FI, c->W, K, c->M1, D, c->ldt, c->Y64);
+    DG_HIP(c, hipGetLastError());
+    return dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy);
+}
+
 int dg_mstep_run(dg_ctx *c) {
     const int M1 = c->M1, D = c->D, K = c->K;
     const long E = (long)M1 * M1 + (long)M1 * D;
@@ -499,10 +507,7 @@ int dg_mstep_run(dg_ctx *c) {
                            ridge, c->Bm, D, c->W, c->scal);
     }
     DG_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_phiw, dim3((unsigned)((K + GT - 1) / GT), (unsigned)((D + GT - 1) / GT)),
-                       dim3(256), 0, c->stream, c->FI, c->W, K, M1, D, c->ldt, c->Y64);
-    DG_HIP(c, hipGetLastError());
-    int rc = dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy);
+    int rc = dg_centres_from_w(c);
     if (rc) return rc;
     // beta partials reuse the (now consumed) A/B partial buffer
     const int nb = (K + kBetaWG - 1) / kBetaWG;

@@ -19,7 +19,7 @@ FP64, MIXED = 0, 1
 SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
-           "dg_set_state", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
+           "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
            "dg_get_timing"]
 
@@ -74,6 +74,7 @@ def load():
     L.dg_upload.argtypes = [vp, _dp, C.c_long, C.c_int, C.c_long, _dp, C.c_int, C.c_int,
                             C.c_long, _dp, C.c_long, C.c_double]
     L.dg_set_state.argtypes = [vp, _dp, C.c_long, C.c_double]
+    L.dg_set_basis.argtypes = [vp, _dp, C.c_int, C.c_int, C.c_long, vp, C.c_long]
     L.dg_estep.argtypes = [vp, C.c_int, C.c_double, C.POINTER(DgEstepOut)]
     L.dg_mstep.argtypes = [vp, C.POINTER(C.c_double)]
     L.dg_anneal.argtypes = [vp, C.POINTER(DgAnnealParams), C.POINTER(DgAnnealResult),
@@ -138,6 +139,17 @@ class Engine:
     def set_state(self, Y, beta):
         Y = _f64(Y)
         self._chk(self.L.dg_set_state(self.h, Y, Y.shape[1], float(beta)))
+
+    def set_basis(self, FI, W=None):
+        """New basis FI (K' x (M+1)); Y = FI W with W given or the last M-step's."""
+        FI = _f64(FI)
+        K, M1 = FI.shape
+        w = None if W is None else _f64(W)
+        if w is not None and w.shape != (M1, self.D):
+            raise ValueError("W must be (M+1) x D")
+        self._chk(self.L.dg_set_basis(self.h, FI, K, M1, M1, None if w is None else w.ctypes.data,
+                                      self.D))
+        self.K = K
 
     def estep(self, isDA=True, temp=1.0):
         o = DgEstepOut()

@@ -34,7 +34,9 @@ struct Opts {
     int L = 3, K = 125, M = 8, maxloop = 0, ckp_every = 100, verbose = 0, device = 0;
     double alpha = 0.99, delta = 0.1, eps = 1e-6, start_temp = 1.0, lambda = 0.0, s = 2.0;
     bool isDA = true, need_ctemp = true, checkpointing = true, log_progress = false;
-    bool save_R = false, mixed = false, init_only = false, interp = false;
+    bool save_R = false, mixed = false, init_only = false, interp = false, hda = false;
+    int Kmax = 32768;
+    double eps2 = 1e-3;
     int schedule = DG_SCHED_EXP;
     unsigned long seed = 0;
     std::string in = "INPUT.h5", out, restart, ckp_prefix = "ckp.";
@@ -51,11 +53,14 @@ void usage(const char *prog) {
                  "-B name      : dataset name for label (default: lb)\n"
                  "-c schedule  : cooling schedule [auto|exp|linear] (default: exp)\n"
                  "-e number    : precision (convergence threshold, default 1e-6)\n"
+                 "-E number    : HDA: least llh gain over a temperature to expand (default 1e-3)\n"
                  "-f dataset   : dataset name for FI (default: FI)\n"
                  "-h           : show help\n"
+                 "-H           : hierarchical DA: double the grid side at critical points\n"
                  "-i filename  : input HDF5 filename (default: INPUT.h5)\n"
                  "-I           : interpolation: MAP of the -i data under the -r model\n"
                  "-j number    : maximum number of iterations\n"
+                 "-k number    : HDA: largest K to expand from (default 32768)\n"
                  "-K number    : number of latent points K (eg, 100, 1k, 1M)\n"
                  "-L number    : latent dimension\n"
                  "-m           : EM (toggle deterministic annealing off)\n"
@@ -107,7 +112,7 @@ int parse(int argc, char **argv, Opts &o) {
                 }
                 break;
             case 'e': o.eps = std::atof(optarg); break;
-            case 'E': break;                      // eps2: used only by HDA
+            case 'E': o.eps2 = std::atof(optarg); break;
             case 'f': o.dFI = optarg; break;
             case 'g':
                 if (strcasecmp(optarg, "sweep")) {
@@ -116,11 +121,11 @@ int parse(int argc, char **argv, Opts &o) {
                 }
                 break;
             case 'h': usage(argv[0]); std::exit(0);
-            case 'H': std::fprintf(stderr, "HDA (-H) is not supported by this engine\n"); return 1;
+            case 'H': o.hda = !o.hda; break;
             case 'i': o.in = optarg; break;
             case 'I': o.interp = true; break;
             case 'j': o.maxloop = std::atoi(optarg); break;
-            case 'k': break;                      // Kmax: used only by HDA
+            case 'k': o.Kmax = std::atoi(optarg); break;
             case 'K': {
                 float num = 0, unit = 1.0f;
                 char ch = 0;
@@ -176,6 +181,25 @@ int parse(int argc, char **argv, Opts &o) {
             return 2;                                                               \
         }                                                                           \
     } while (0)
+
+// The state dagtm.cxx:1983-2095 appends to the output file at an HDA
+// expansion (with -p): M.<idx> = R^T X, X, Y, FI, W, the per-centre critical
+// temperatures of the last search and beta, idx = progress entries - 1.  W0
+// is the host's initial W, written while no M-step has run.
+bool write_expansion_snapshot(dg_ctx *ctx, const Opts &o, long idx, long N, int D, const Mat &X,
+                              const Mat &FI, const Mat *W0, const std::vector<double> &ctemp_k,
+                              double beta) {
+    const int K = (int)FI.rows;
+    Mat Mo(N, X.cols), Yc(K, D), Wc(FI.cols, D);
+    if (dg_download(ctx, W0 ? nullptr : Wc.data(), D, Yc.data(), D, nullptr) != DG_OK) return false;
+    if (dg_map(ctx, X.data(), (int)X.cols, X.cols, Mo.data(), X.cols) != DG_OK) return false;
+    const std::string sfx = "." + std::to_string(idx);
+    H5File f = H5File::open_append(o.out);
+    return f.ok() && f.write_matrix("M" + sfx, Mo) && f.write_matrix("X" + sfx, X) &&
+           f.write_matrix("Y" + sfx, Yc) && f.write_matrix("FI" + sfx, FI) &&
+           f.write_matrix("W" + sfx, W0 ? *W0 : Wc) && f.write_vector("ctemp" + sfx, ctemp_k) &&
+           f.write_scalar("beta" + sfx, beta);
+}
 
 }  // namespace
 
@@ -259,6 +283,13 @@ int main(int argc, char **argv) {
         return ok ? 0 : 1;
     }
 
+    // ---- the output file is created before any work (dagtm.cxx:1216-1220);
+    // HDA snapshots, the -I projection and the final state are appended
+    if (!o.out.empty()) {
+        H5File f = H5File::create(o.out);
+        if (!f.ok()) { std::fprintf(stderr, "cannot create %s\n", o.out.c_str()); return 1; }
+    }
+
     // ---- the engine
     dg_ctx *ctx = nullptr;
     dg_config cfg{o.device, o.mixed ? DG_MIXED : DG_FP64, o.lambda};
@@ -297,18 +328,25 @@ int main(int argc, char **argv) {
         return ok ? 0 : 1;
     }
 
-    // ---- annealing loop (dagtm.cxx:1508-2206, AUTO / EXP / LINEAR / EM)
+    // ---- annealing loop (dagtm.cxx:1508-2206, AUTO / EXP / LINEAR / EM, HDA)
     const int QN = 50;
     std::vector<double> Q(QN);
     std::vector<double> hT, hL, hF;
     double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff = 0;
     bool isLast = false;
+    // HDA state (dagtm.cxx:1504-1506): an armed expansion fires at the end of
+    // the next temperature if the llh gained over it is at least eps2
+    bool ready = false, do_expand = false, w_in_engine = false;
+    int nexpanded = 0;
+    std::vector<double> ctemp_k;
+    const bool fresh = o.restart.empty();
     auto absv = [](double a) { return (a > 0) ? a : -a; };
     do {
         if (temp < 1.0) { temp = 1.0; isLast = true; }
         std::printf("\nT= %.10g\n", temp);
         std::fill(Q.begin(), Q.end(), DBL_MAX);
         int qnext = 0, lloop = 0;
+        const double qual_before_temp = llh;
         std::printf(">>> %15s %15s %15s %15s %15s %15s\n", "GSEQ", "SEQ", "K", "TEMP", "NFREE", "LLH");
         while (true) {
             dg_estep_out eo;
@@ -345,6 +383,7 @@ int main(int argc, char **argv) {
                 if (absv(avg - old) < o.eps) break;
             }
             CHECK_DG(dg_mstep(ctx, &beta));
+            w_in_engine = true;
             if (o.maxloop && nloop >= o.maxloop) { isLast = true; break; }
         }
         if (o.isDA && !isLast) {
@@ -356,6 +395,45 @@ int main(int argc, char **argv) {
                 CHECK_DG(dg_next_ctemp(ctx, temp, o.eps, nullptr, &ct, nullptr));
                 temp = (ct >= temp) ? temp * o.alpha : ct;
             } else {
+                if (o.hda && K < o.Kmax) {
+                    // dagtm.cxx:1912, 1939-1977
+                    const double qual_diff_at_temp = llh - qual_before_temp;
+                    if (ready) {
+                        if (qual_diff_at_temp < o.eps2) ready = false;
+                        else do_expand = true;
+                    }
+                    if (!ready) {
+                        double ct = 0.0;
+                        ctemp_k.assign(K, 0.0);
+                        CHECK_DG(dg_next_ctemp(ctx, temp, o.eps, nullptr, &ct, ctemp_k.data()));
+                        if (ct >= temp * o.alpha) ready = true;
+                    }
+                    if (do_expand) {
+                        do_expand = ready = false;
+                        ++nexpanded;
+                        if (!o.out.empty() && o.log_progress &&
+                            !write_expansion_snapshot(ctx, o, (long)hT.size() - 1, N, D, X, FI,
+                                                      w_in_engine ? nullptr : &W, ctemp_k, beta))
+                            std::fprintf(stderr, "HDA snapshot write failed\n");
+                        // dagtm.cxx:2101-2160: double the grid side, new grid and
+                        // basis, Y = FI W with the current W
+                        K = (int)std::lround(std::pow(std::lround(std::pow(K, 1.0 / o.L)) * 2.0, o.L));
+                        if (!dgh::setup_basis(o.L, K, o.M, o.s, X, FI)) {
+                            std::fprintf(stderr, "HDA: no grid for K=%d M=%d L=%d\n", K, o.M, o.L);
+                            dg_destroy(ctx);
+                            return 2;
+                        }
+                        if (!w_in_engine && !fresh) {
+                            std::fprintf(stderr, "HDA: expansion before any M-step after a restart\n");
+                            dg_destroy(ctx);
+                            return 2;
+                        }
+                        CHECK_DG(dg_set_basis(ctx, FI.data(), K, (int)FI.cols, FI.cols,
+                                              w_in_engine ? nullptr : W.data(), D));
+                        w_in_engine = true;
+                        std::printf("new K : %d\n", K);
+                    }
+                }
                 temp *= o.alpha;
             }
         } else {
@@ -365,6 +443,7 @@ int main(int argc, char **argv) {
 
     std::printf("\nSummary: iter %d temp %.10g beta %.10g llh %.10g nfree %.10g\n", nloop, temp,
                 beta, llh, nfree);
+    if (o.hda) std::printf("HDA: %d expansions, K = %d\n", nexpanded, K);
 
     // ---- output (dagtm.cxx:2342-2552)
     if (!o.out.empty()) {
@@ -372,7 +451,7 @@ int main(int argc, char **argv) {
         double b = 0;
         CHECK_DG(dg_download(ctx, Wo.data(), D, Yo.data(), D, &b));
         CHECK_DG(dg_map(ctx, X.data(), (int)X.cols, X.cols, Mo.data(), X.cols));
-        H5File f = H5File::create(o.out);
+        H5File f = H5File::open_append(o.out);
         bool ok = f.ok() && f.write_matrix(o.dT, T) && f.write_matrix(o.dM, Mo) &&
                   f.write_matrix(o.dX, X) && f.write_matrix(o.dFI, FI) && f.write_matrix(o.dY, Yo) &&
                   f.write_matrix(o.dW, Wo) && f.write_scalar(o.dbeta, b) &&

@@ -64,6 +64,16 @@ int dg_upload(dg_ctx *ctx, const double *T, long N, int D, long ldT,
 /* Replaces the restart read of Y and beta (dagtm.cxx:1161-1213). */
 int dg_set_state(dg_ctx *ctx, const double *Y, long ldY, double beta);
 
+/* New latent basis with the same M + 1, as in the HDA expansion: replaces
+ * the re-set-up of dagtm.cxx:2112-2160 (new FI, re-allocation of every
+ * K-sized buffer, Y = FI W, dist).  FI is K x M1 (ldFI >= M1, M1 must equal
+ * the uploaded M + 1); W (M1 x D, ldW >= D) may be NULL to keep the W of the
+ * last dg_mstep.  T stays resident; beta is unchanged; the G of the last
+ * M-step (used by dg_next_ctemp) restarts at zero; a dg_estep is needed
+ * before the next dg_mstep. */
+int dg_set_basis(dg_ctx *ctx, const double *FI, int K, int M1, long ldFI,
+                 const double *W, long ldW);
+
 typedef struct {
     double llh;        /* normalised log-likelihood (mpicomm.c:463-500, from g) */
     double nfree;      /* normalised -F (from h; equals -llh in EM mode)        */

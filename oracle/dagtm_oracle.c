@@ -653,12 +653,13 @@ static void q_push(qqueue *q, double x) { q->v[q->next] = x; q->next = (q->next 
 static double q_mean(const qqueue *q) { double s = 0; for (int i = 0; i < QUEUE_SIZE; i++) s += q->v[i]; return s / QUEUE_SIZE; }
 #define ABSV(a) (((a) > (0)) ? (a) : (-(a)))
 
-/* dagtm.cxx:1508-1795 + 1909-2206 for the EXP and LINEAR schedules and EM
- * mode: clamp temp<1 to 1 and mark the last pass (1510-1514); reset the
- * quality queue per temperature (1521); per iteration resp, qual, the
+/* dagtm.cxx:1508-1795 + 1909-2206 for the AUTO, EXP and LINEAR schedules
+ * and EM mode: clamp temp<1 to 1 and mark the last pass (1510-1514); reset
+ * the quality queue per temperature (1521); per iteration resp, qual, the
  * |diff|<eps test and the 50-window moving-average test (1689-1709), both
- * breaking before the M-step; M-step + beta; maxloop (1790-1795); then
- * temp *= alpha or temp -= delta (2166-2171).  qual_old starts at 0 (Q7). */
+ * breaking before the M-step; M-step + beta; maxloop (1790-1795); then the
+ * next critical temperature (1919-1935), temp *= alpha or temp -= delta
+ * (2166-2171).  qual_old starts at 0 (Q7). */
 int orc_anneal(orc_model *m, const orc_anneal_params *p,
                orc_anneal_result *res, orc_iter_cb cb, void *user)
 {

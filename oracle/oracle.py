@@ -8,6 +8,7 @@ Every wrapper takes and returns C-contiguous float64 numpy arrays.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -309,3 +310,89 @@ class Model:
         Mo = np.zeros((self.m.N, X.shape[1]))
         lib().orc_map(C.byref(self.m), X, X.shape[1], Mo)
         return Mo
+
+
+def anneal_hda(T, FI, Y, beta, W, L, M, s, Kmax=32768, eps2=1e-3, alpha=0.99, eps=1e-6,
+               start_temp=1.0, maxloop=0, lam=0.0, solver=SOLVE_PINV, X=None, on_expand=None):
+    """Hierarchical DA: the EXP loop of dagtm.cxx:1508-1795, 1909-2164 with -H.
+
+    After each temperature (not the last) and while K < Kmax: if an expansion
+    was armed, it fires when the llh gained over the temperature
+    (qual_diff_at_temp, 1912) is at least eps2 and is cancelled otherwise; an
+    unarmed state computes the next critical temperature and arms when it is
+    >= temp * alpha (1944-1970).  Firing doubles the grid side,
+    K <- (2 round(K^(1/L)))^L, builds the new grid and basis with the same M and
+    width s, and sets Y = FI W with the last M-step's W (W0 before any M-step,
+    2101-2160).  Then temp *= alpha.  on_expand(idx, M, X, Y, FI, W, ctemp,
+    beta) receives the state the reference logs at an expansion (1983-2095),
+    idx = nloop - 1.  Returns (summary dict, rows), each row (nloop, lloop, K,
+    temp, nfree, llh, diff, beta)."""
+    T = _f64(T)
+    N, D = T.shape
+    K = FI.shape[0]
+    if X is None:
+        X, _ = stp(L, K, M, s)
+    m = Model(T, FI, Y, beta, lam=lam, solver=solver)
+    Wlast = _f64(W).copy()
+    temp = start_temp
+    isLast = False
+    nloop = ntemps = nexp = 0
+    qual = llh = nfree = 0.0
+    ready = do_expand = False
+    ctemp_k = None
+    rows = []
+    QN = 50
+    while True:
+        if temp < 1.0:
+            temp, isLast = 1.0, True
+        Q = [sys.float_info.max] * QN
+        qn = 0
+        ntemps += 1
+        qual_before = llh
+        lloop = 0
+        while True:
+            llh, nfree, _ = m.estep(1, temp)
+            qual_old = qual
+            lloop += 1
+            nloop += 1
+            qual = nfree
+            diff = qual - qual_old
+            rows.append((nloop, lloop, K, temp, nfree, llh, diff, m.beta))
+            if abs(diff) < eps:
+                break
+            if nloop > QN:
+                old = sum(Q) / QN
+                Q[qn] = qual
+                qn = (qn + 1) % QN
+                if abs(sum(Q) / QN - old) < eps:
+                    break
+            m.mstep()
+            Wlast = m.W.copy()
+            if maxloop and nloop >= maxloop:
+                isLast = True
+                break
+        if isLast:
+            break
+        qdiff = llh - qual_before
+        if K < Kmax:
+            if ready:
+                if qdiff < eps2:
+                    ready = False
+                else:
+                    do_expand = True
+            if not ready:
+                c, ctemp_k = m.next_ctemp(temp, eps)
+                if c >= temp * alpha:
+                    ready = True
+            if do_expand:
+                do_expand = ready = False
+                nexp += 1
+                if on_expand is not None:
+                    on_expand(nloop - 1, m.map(X), X, m.Y.copy(), FI, Wlast, ctemp_k, m.beta)
+                g = int(round(K ** (1.0 / L))) * 2
+                K = g ** L
+                X, FI = stp(L, K, M, s)
+                m = Model(T, FI, FI @ Wlast, m.beta, lam=lam, solver=solver)
+        temp *= alpha
+    return dict(nloop=nloop, ntemps=ntemps, nexp=nexp, K=K, temp=temp, llh=llh, nfree=nfree,
+                beta=m.beta, Y=m.Y.copy(), X=X, FI=FI, W=Wlast), rows

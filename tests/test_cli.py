@@ -47,6 +47,6 @@ def test_init_chain_matches_oracle(tmp_path, oil_state, K, M, L, seed, extra):
 
 
 def test_cli_rejects_unsupported_options(tmp_path):
-    for args in (["-c", "bogus"], ["-H"], ["-g", "spacefilling"], ["-Q"]):
+    for args in (["-c", "bogus"], ["-g", "spacefilling"], ["-Q"]):
         r = subprocess.run([DAGTM] + args, capture_output=True, text=True)
         assert r.returncode == 1

@@ -110,6 +110,64 @@ def test_cli_auto_schedule_matches_oracle(tmp_path, oil_state, oil_input):
         assert abs(g[4] - o[4]) <= tol, (g[0], g[4], o[4], tol)
 
 
+def test_cli_hda_matches_oracle(tmp_path, oil_state, oil_input):
+    _say("case: dagtm -H on the Oil data, K=25 L=2 M=9, 150 iterations, fp64")
+    init = str(tmp_path / "init.h5")
+    common = ["-i", oil_input, "-K", "25", "-M", "9", "-L", "2", "-s", "5", "-a", "0.9"]
+    subprocess.run([DAGTM, "--init-only", "-o", init] + common, check=True)
+    out = str(tmp_path / "out.h5")
+    r = subprocess.run([DAGTM, "-n", "-j", "150", "-H", "-k", "300", "-E", "1e-3", "-p", "-o", out]
+                       + common, check=True, capture_output=True, text=True, cwd=tmp_path,
+                       timeout=300)
+    rows = _rows(r.stdout)
+    T = oil_state["T"]
+    rd = lambda n, sh: h5util.read(init, n).reshape(sh)
+    X0, FI, Y0, W0 = rd("X", (25, 2)), rd("FI", (25, 10)), rd("Y", (25, 12)), rd("W", (10, 12))
+    b0, t0 = h5util.read(init, "beta")[0], h5util.read(init, "temp")[0]
+    kw = dict(Kmax=300, eps2=1e-3, alpha=0.9, eps=1e-6, start_temp=t0, maxloop=150,
+              solver=O.SOLVE_CHOL, X=X0)
+    snaps = []
+    res, orows = O.anneal_hda(T, FI, Y0, b0, W0, 2, 9, 2.0, on_expand=lambda *a: snaps.append(a),
+                              **kw)
+    # Past the phase transition near iteration 49 the run is chaotic: oracle runs
+    # from Y0 (1 + 1e-15 .. 1e-13) differ from it by 1e-7 .. 1e-4 in llh, about
+    # independently of the perturbation's size.  The envelope of three such runs
+    # over a +-5 row window, times 10, bounds what the GPU may differ by.
+    pert = []
+    for dp in (1e-15, 1e-14, 1e-13):
+        ps = []
+        _, prows = O.anneal_hda(T, FI, Y0 * (1.0 + dp), b0, W0, 2, 9, 2.0,
+                                on_expand=lambda *a: ps.append(a), **kw)
+        assert [p[2] for p in prows] == [o[2] for o in orows]      # same expansion point
+        pert.append((prows, ps[0]))
+    ks = [int(line.split()[3]) for line in r.stdout.splitlines() if line.startswith(":::")]
+    assert res["nexp"] == 1 and len(rows) == len(ks) == len(orows) == 150
+    dev = np.array([[abs(p[i][5] - orows[i][5]) for i in range(150)] for p, _ in pert]).max(0)
+    env = np.array([dev[max(0, i - 5):i + 6].max() for i in range(150)])
+    # g: (nloop, lloop, temp, nfree, llh, diff); o: (nloop, lloop, K, temp, nfree, llh, ...)
+    for i, (g, k, o) in enumerate(zip(rows, ks, orows)):
+        assert g[0] == o[0] and g[1] == o[1] and k == o[2]                 # nloop, lloop, K
+        assert abs(g[2] - o[3]) <= 1e-7 + 1e-7 * o[3], (g[0], g[2], o[3])  # temp
+        tol = 2e-7 + 1e-7 * abs(o[5]) + 10.0 * env[i]                     # llh (printed %15.7f)
+        assert abs(g[4] - o[5]) <= tol, (g[0], g[4], o[5], tol)
+    assert "new K : 100" in r.stdout
+    # the expansion snapshot (dagtm.cxx:1983-2095) and the final state at K = 100
+    idx, Mo, Xs, Ys, FIs, Ws, ck, beta = snaps[0]
+    assert all(ps[0] == idx for _, ps in pert)
+    sfx = ".%d" % idx
+    assert np.abs(h5util.read(out, "X" + sfx).reshape(25, 2) - Xs).max() < 1e-15
+    assert np.abs(h5util.read(out, "FI" + sfx).reshape(25, 10) - FIs).max() < 1e-14
+    for name, shp, pos in (("M", (1000, 2), 1), ("Y", (25, 12), 3), ("W", (10, 12), 5),
+                           ("ctemp", (25,), 6), ("beta", (1,), 7)):
+        ref = np.atleast_1d(np.asarray(snaps[0][pos], dtype=float))
+        spread = max(np.abs(np.atleast_1d(np.asarray(ps[pos], dtype=float)) - ref).max()
+                     for _, ps in pert)
+        tol = 1e-8 * np.abs(ref).max() + 10.0 * spread
+        assert np.abs(h5util.read(out, name + sfx).reshape(shp) - ref).max() <= tol, name
+    for name, shp in (("X", (100, 2)), ("FI", (100, 10)), ("Y", (100, 12)), ("M", (1000, 2))):
+        assert h5util.shape(out, name) == shp, name
+
+
 def test_checkpoint_restart_reproduces_trajectory(tmp_path, oil_input):
     _say("case: dagtm checkpoint every 10, restart from iteration 20")
     common = ["-i", oil_input, "-K", "125", "-M", "8", "-L", "3", "-s", "5", "-e", "0",
