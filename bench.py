@@ -133,7 +133,11 @@ def main():
     e16.upload(T, FI, Y0, b0)
     e16.set_timing(True)
     print("[bench] mixed warmup", file=sys.stderr, flush=True)
-    _, llh16_w, _ = run_steps(e16, args.warmup)
+    # the state after the first iteration is compared with the FP64 path's
+    _, llh16_w, _ = run_steps(e16, 1)
+    st16 = e16.download()
+    _, llh_more, _ = run_steps(e16, max(0, args.warmup - 1))
+    llh16_w += llh_more
     print("[bench] mixed timed", file=sys.stderr, flush=True)
     sec16, llh16_t, ph16 = run_steps(e16, args.steps)
     llh16 = llh16_w + llh16_t
@@ -146,6 +150,7 @@ def main():
     e64.set_timing(True)
     print("[bench] fp64", file=sys.stderr, flush=True)
     _, llh64_w, _ = run_steps(e64, 1)
+    st64 = e64.download()
     sec64, llh64_t, ph64 = run_steps(e64, args.fp64_steps)
     llh64 = llh64_w + llh64_t
     it64 = args.fp64_steps / sec64
@@ -153,6 +158,10 @@ def main():
 
     n = min(len(llh16), len(llh64))
     llh_err = max(abs(a - b) / abs(b) for a, b in zip(llh16[:n], llh64[:n]))
+    # W, Y (max-norm relative) and beta after one iteration from the same state
+    state_err = {"W": float(np.abs(st16[0] - st64[0]).max() / np.abs(st64[0]).max()),
+                 "Y": float(np.abs(st16[1] - st64[1]).max() / np.abs(st64[1]).max()),
+                 "beta": abs(st16[2] - st64[2]) / st64[2]}
 
     # ---- roofline of the dominant kernel (per launch averages from HIP events)
     p1_ms, p2_ms = ph16[0] / args.steps, ph16[1] / args.steps
@@ -185,6 +194,7 @@ def main():
         "fp64_gpu_iter_per_s": round(it64, 4),
         "mixed_over_fp64": round(it16 / it64, 2),
         "llh_rel_err_vs_fp64": llh_err,
+        "state_rel_err_vs_fp64_after_1_iter": state_err,
         "kd_updates_per_s": K * N * D * it16,
         "phase_ms_mixed": [round(x / args.steps, 4) for x in ph16],
         "phase_ms_fp64": [round(x / args.fp64_steps, 4) for x in ph64],
