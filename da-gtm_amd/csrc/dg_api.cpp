@@ -487,6 +487,48 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
     return DG_OK;
 }
 
+int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *items, int n,
+                    dg_anneal_result *res, int *status) {
+    if (!cfg || n < 0 || (n > 0 && (!items || !res || !status))) return DG_EINVAL;
+    if (n == 0) return DG_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) {
+        (void)hipGetLastError();
+        return DG_EHIP;
+    }
+    if (workers <= 0) workers = 4;
+    workers = std::min(workers, n);
+    // largest work first, so the tail of the queue holds the short items
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double)items[a].K * items[a].N * items[a].D >
+               (double)items[b].K * items[b].N * items[b].D;
+    });
+    for (int i = 0; i < n; ++i) { status[i] = DG_EHIP; std::memset(&res[i], 0, sizeof(res[i])); }
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        dg_ctx *c = nullptr;
+        if (dg_create(cfg, &c) != DG_OK) return;   // the other workers take its share
+        for (int j; (j = next.fetch_add(1)) < n;) {
+            const int i = order[j];
+            const dg_batch_item &it = items[i];
+            int rc = dg_upload(c, it.T, it.N, it.D, it.ldT, it.FI, it.K, it.M1, it.ldFI, it.Y0,
+                               it.ldY0, it.beta0);
+            if (!rc) rc = dg_anneal(c, &it.params, &res[i], nullptr, nullptr);
+            if (!rc && (it.W || it.Y))
+                rc = dg_download(c, it.W, it.ldW, it.Y, it.ldY, nullptr);
+            status[i] = rc;
+        }
+        dg_destroy(c);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < workers; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    return DG_OK;
+}
+
 int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
     if (!c) return DG_EINVAL;
     if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_map before dg_upload");

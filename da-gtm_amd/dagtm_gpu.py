@@ -21,7 +21,7 @@ SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
-           "dg_get_timing"]
+           "dg_get_timing", "dg_batch_anneal"]
 
 
 class DgConfig(C.Structure):
@@ -42,6 +42,14 @@ class DgAnnealResult(C.Structure):
     _fields_ = [("nloop", C.c_int), ("ntemps", C.c_int), ("exit_maxloop", C.c_int),
                 ("temp", C.c_double), ("llh", C.c_double), ("nfree", C.c_double),
                 ("qual", C.c_double), ("beta", C.c_double)]
+
+
+class DgBatchItem(C.Structure):
+    _fields_ = [("T", C.c_void_p), ("N", C.c_long), ("D", C.c_int), ("ldT", C.c_long),
+                ("FI", C.c_void_p), ("K", C.c_int), ("M1", C.c_int), ("ldFI", C.c_long),
+                ("Y0", C.c_void_p), ("ldY0", C.c_long), ("beta0", C.c_double),
+                ("params", DgAnnealParams),
+                ("W", C.c_void_p), ("ldW", C.c_long), ("Y", C.c_void_p), ("ldY", C.c_long)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -86,6 +94,8 @@ def load():
     L.dg_get_stats.argtypes = [vp, vp, vp, C.c_long]
     L.dg_set_timing.argtypes = [vp, C.c_int]
     L.dg_get_timing.argtypes = [vp, C.POINTER(C.c_double)]
+    L.dg_batch_anneal.argtypes = [C.POINTER(DgConfig), C.c_int, C.POINTER(DgBatchItem), C.c_int,
+                                  C.POINTER(DgAnnealResult), C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -217,3 +227,44 @@ class Engine:
         t = (C.c_double * 4)()
         self._chk(self.L.dg_get_timing(self.h, t))
         return list(t)
+
+
+def batch_anneal(problems, precision=FP64, lam=0.0, workers=4, device=0, **anneal_kw):
+    """Anneal independent instances concurrently (dg_batch_anneal).
+
+    problems: list of (T, FI, Y0, beta0) or (T, FI, Y0, beta0, params_dict); the
+    keyword arguments are the default annealing parameters (as Engine.anneal).
+    Returns a list of (status, DgAnnealResult, W, Y)."""
+    L = load()
+    n = len(problems)
+    items = (DgBatchItem * n)()
+    res = (DgAnnealResult * n)()
+    status = (C.c_int * n)()
+    keep = []
+    outs = []
+    for i, p in enumerate(problems):
+        T, FI, Y0, b0 = (_f64(p[0]), _f64(p[1]), _f64(p[2]), float(p[3]))
+        kw = dict(isDA=True, schedule=SCHED_EXP, maxloop=0, alpha=0.99, delta=0.1, eps=1e-6,
+                  start_temp=1.0)
+        kw.update(anneal_kw)
+        if len(p) > 4:
+            kw.update(p[4])
+        N, D = T.shape
+        K, M1 = FI.shape
+        W = np.zeros((M1, D))
+        Y = np.zeros((K, D))
+        keep += [T, FI, Y0]
+        outs.append((W, Y))
+        items[i] = DgBatchItem(T=T.ctypes.data, N=N, D=D, ldT=D, FI=FI.ctypes.data, K=K, M1=M1,
+                               ldFI=M1, Y0=Y0.ctypes.data, ldY0=D, beta0=b0,
+                               params=DgAnnealParams(isDA=int(kw["isDA"]),
+                                                     schedule=kw["schedule"],
+                                                     maxloop=kw["maxloop"], alpha=kw["alpha"],
+                                                     delta=kw["delta"], eps=kw["eps"],
+                                                     start_temp=kw["start_temp"]),
+                               W=W.ctypes.data, ldW=D, Y=Y.ctypes.data, ldY=D)
+    cfg = DgConfig(device=device, precision=precision, lambda_=lam)
+    rc = L.dg_batch_anneal(C.byref(cfg), int(workers), items, n, res, status)
+    if rc:
+        raise DgError(rc, "dg_batch_anneal could not start")
+    return [(status[i], res[i], outs[i][0], outs[i][1]) for i in range(n)]
