@@ -8,6 +8,8 @@
 #include <cstring>
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <functional>
 #include <new>
 #include <string>
 #include <thread>
@@ -487,16 +489,22 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
     return DG_OK;
 }
 
+constexpr int kMaxBatchWorkers = 4;
+
 int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *items, int n,
-                    dg_anneal_result *res, int *status) {
+                    dg_anneal_result *res, int *status, double *phase_ms) {
     if (!cfg || n < 0 || (n > 0 && (!items || !res || !status))) return DG_EINVAL;
+    if (phase_ms) phase_ms[0] = phase_ms[1] = 0.0;
     if (n == 0) return DG_OK;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) {
         (void)hipGetLastError();
         return DG_EHIP;
     }
-    if (workers <= 0) workers = 4;
+    // at most kMaxBatchWorkers threads: one per hardware queue of the process
+    // (HIP's default of four); a run with eight workers and every instance
+    // resident stopped responding once, with no cause found in this code
+    if (workers <= 0 || workers > kMaxBatchWorkers) workers = kMaxBatchWorkers;
     workers = std::min(workers, n);
     // largest work first, so the tail of the queue holds the short items
     std::vector<int> order(n);
@@ -506,26 +514,44 @@ int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *item
                (double)items[b].K * items[b].N * items[b].D;
     });
     for (int i = 0; i < n; ++i) { status[i] = DG_EHIP; std::memset(&res[i], 0, sizeof(res[i])); }
-    std::atomic<int> next{0};
-    auto worker = [&]() {
-        dg_ctx *c = nullptr;
-        if (dg_create(cfg, &c) != DG_OK) return;   // the other workers take its share
-        for (int j; (j = next.fetch_add(1)) < n;) {
-            const int i = order[j];
-            const dg_batch_item &it = items[i];
-            int rc = dg_upload(c, it.T, it.N, it.D, it.ldT, it.FI, it.K, it.M1, it.ldFI, it.Y0,
-                               it.ldY0, it.beta0);
-            if (!rc) rc = dg_anneal(c, &it.params, &res[i], nullptr, nullptr);
-            if (!rc && (it.W || it.Y))
-                rc = dg_download(c, it.W, it.ldW, it.Y, it.ldY, nullptr);
-            status[i] = rc;
-        }
-        dg_destroy(c);
+    std::vector<dg_ctx *> ctx(n, nullptr);
+    auto run_pool = [&](const std::function<void(int)> &job) {
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int j; (j = next.fetch_add(1)) < n;) job(order[j]);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < workers; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
     };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < workers; ++t) pool.emplace_back(worker);
-    worker();
-    for (auto &th : pool) th.join();
+    using clk = std::chrono::steady_clock;
+    // phase 1: one context per instance, data resident on the device
+    const auto t0 = clk::now();
+    run_pool([&](int i) {
+        const dg_batch_item &it = items[i];
+        int rc = dg_create(cfg, &ctx[i]);
+        if (!rc) rc = dg_upload(ctx[i], it.T, it.N, it.D, it.ldT, it.FI, it.K, it.M1, it.ldFI,
+                                it.Y0, it.ldY0, it.beta0);
+        status[i] = rc;
+    });
+    // phase 2: the annealing runs, concurrently on the workers' host threads
+    const auto t1 = clk::now();
+    run_pool([&](int i) {
+        if (status[i] != DG_OK) return;
+        status[i] = dg_anneal(ctx[i], &items[i].params, &res[i], nullptr, nullptr);
+    });
+    const auto t2 = clk::now();
+    for (int i = 0; i < n; ++i) {
+        const dg_batch_item &it = items[i];
+        if (status[i] == DG_OK && (it.W || it.Y))
+            status[i] = dg_download(ctx[i], it.W, it.ldW, it.Y, it.ldY, nullptr);
+        if (ctx[i]) dg_destroy(ctx[i]);
+    }
+    if (phase_ms) {
+        phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        phase_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    }
     return DG_OK;
 }
 

@@ -95,7 +95,8 @@ def load():
     L.dg_set_timing.argtypes = [vp, C.c_int]
     L.dg_get_timing.argtypes = [vp, C.POINTER(C.c_double)]
     L.dg_batch_anneal.argtypes = [C.POINTER(DgConfig), C.c_int, C.POINTER(DgBatchItem), C.c_int,
-                                  C.POINTER(DgAnnealResult), C.POINTER(C.c_int)]
+                                  C.POINTER(DgAnnealResult), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -234,7 +235,7 @@ def batch_anneal(problems, precision=FP64, lam=0.0, workers=4, device=0, **annea
 
     problems: list of (T, FI, Y0, beta0) or (T, FI, Y0, beta0, params_dict); the
     keyword arguments are the default annealing parameters (as Engine.anneal).
-    Returns a list of (status, DgAnnealResult, W, Y)."""
+    Returns (list of (status, DgAnnealResult, W, Y), [upload ms, anneal ms])."""
     L = load()
     n = len(problems)
     items = (DgBatchItem * n)()
@@ -264,7 +265,8 @@ def batch_anneal(problems, precision=FP64, lam=0.0, workers=4, device=0, **annea
                                                      start_temp=kw["start_temp"]),
                                W=W.ctypes.data, ldW=D, Y=Y.ctypes.data, ldY=D)
     cfg = DgConfig(device=device, precision=precision, lambda_=lam)
-    rc = L.dg_batch_anneal(C.byref(cfg), int(workers), items, n, res, status)
+    ph = (C.c_double * 2)()
+    rc = L.dg_batch_anneal(C.byref(cfg), int(workers), items, n, res, status, ph)
     if rc:
         raise DgError(rc, "dg_batch_anneal could not start")
-    return [(status[i], res[i], outs[i][0], outs[i][1]) for i in range(n)]
+    return [(status[i], res[i], outs[i][0], outs[i][1]) for i in range(n)], list(ph)

@@ -144,12 +144,13 @@ int dg_next_ctemp(dg_ctx *ctx, double temp, double eps, const double *gg,
                   double *ctemp_max, double *ctemp_k);
 
 /* ---- many independent instances ------------------------------------------
- * One annealing run per item (any N, D, K, M per item), on one GPU.  A pool
- * of `workers` host threads, each owning one context and one HIP stream,
- * takes items from a shared queue in order of decreasing K*N*D, so kernels of
- * different instances run concurrently.  Results do not depend on the
- * worker count or order (every kernel is deterministic).  The reference runs
- * one instance per process; this has no counterpart there. */
+ * One annealing run per item (any N, D, K, M per item), on one GPU.  Every
+ * item gets its own context (and HIP stream) with its data resident on the
+ * device; then a pool of `workers` host threads takes the items from a shared
+ * queue in order of decreasing K*N*D and runs their annealing loops, so
+ * kernels of different instances run concurrently.  Results do not depend on
+ * the worker count or order (every kernel is deterministic).  The reference
+ * runs one instance per process; this has no counterpart there. */
 typedef struct {
     const double *T;  long N; int D; long ldT;     /* data, N x D */
     const double *FI; int K; int M1; long ldFI;    /* basis, K x M1 */
@@ -160,13 +161,13 @@ typedef struct {
     double *Y; long ldY;                           /* out: K x D, may be NULL */
 } dg_batch_item;
 
-/* cfg: device, precision and lambda shared by all items; workers <= 0 means
- * 4 (one per hardware queue).  res[i] and status[i] (DG_OK or the item's
- * error code; a failing item does not stop the others) receive each item's
- * outcome.  Returns DG_OK, or DG_EINVAL / DG_EHIP when the batch could not
- * start at all. */
+/* cfg: device, precision and lambda shared by all items; workers is clamped
+ * to 1..4 (<= 0 means 4).  res[i] and status[i] (DG_OK or the item's error code; a failing item
+ * does not stop the others) receive each item's outcome; phase_ms (may be
+ * NULL) the wall time of the upload phase and of the annealing phase.
+ * Returns DG_OK, or DG_EINVAL / DG_EHIP when the batch could not start. */
 int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *items, int n,
-                    dg_anneal_result *res, int *status);
+                    dg_anneal_result *res, int *status, double *phase_ms);
 
 /* MAP projection M = R^T X (N x L, ldM >= L) with the responsibilities at
  * the current model state and the last E-step's temperature (an E-step is

@@ -41,7 +41,8 @@ def test_batch_equals_sequential(prec, workers):
     _say(f"case: batch of {len(CASES)} instances, precision {prec}, {workers} workers")
     probs = _instances()
     kw = dict(alpha=0.9, eps=1e-6)
-    out = dg.batch_anneal(probs, precision=prec, lam=1e-3, workers=workers, **kw)
+    out, ph = dg.batch_anneal(probs, precision=prec, lam=1e-3, workers=workers, **kw)
+    assert ph[0] > 0 and ph[1] > 0
     for i, (p, (st, r, W, Y)) in enumerate(zip(probs, out)):
         assert st == dg.DG_OK, (i, st)
         with dg.Engine(precision=prec, lam=1e-3) as e:
@@ -60,6 +61,6 @@ def test_batch_item_errors_are_per_item():
     T, FI, Y0, b0, kw = probs[1]
     Tbad = np.zeros((50, 200))                   # D > 192 is rejected by dg_upload
     probs.insert(1, (Tbad, FI, np.zeros((FI.shape[0], 200)), 1.0, kw))
-    out = dg.batch_anneal(probs, precision=dg.FP64, workers=2, alpha=0.9, eps=1e-6)
+    out, _ = dg.batch_anneal(probs, precision=dg.FP64, workers=2, alpha=0.9, eps=1e-6)
     assert [o[0] for o in out] == [dg.DG_OK, dg.DG_EINVAL, dg.DG_OK, dg.DG_OK]
     assert all(o[1].nloop > 0 for i, o in enumerate(out) if i != 1)
