@@ -720,21 +720,46 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ---------------------------------------------------------------- helpers
-__global__ void k_reduce32(const float *__restrict__ partRT, const float *__restrict__ partG,
-                           int NC, int K, int D, double *__restrict__ RT,
-                           double *__restrict__ G) {
+// RT[k][d] = sum_c partRT[c][d][k] and G[k] = sum_c partG[c][k], each in FP64
+// in chunk order.  One thread per output (lanes along the contiguous centre
+// axis); the chunk loop issues eight independent loads before it adds them,
+// in order.  A workgroup owns 64 centres x 16 dims and transposes them through
+// LDS so each FP64 row segment of RT is written as 128 contiguous bytes.
+// Workgroups of the first dim tile also sum G.
+constexpr int RK = 64, RD = 16, RU = 8;
+
+__global__ __launch_bounds__(RK * RD) void k_reduce32(const float *__restrict__ partRT,
+                                                      const float *__restrict__ partG, int NC,
+                                                      int K, int D, double *__restrict__ RT,
+                                                      double *__restrict__ G) {
+    __shared__ double tile[RK][RD + 1];
+    const int t = threadIdx.x, kk = t & (RK - 1), dd = t / RK;
+    const int k0 = blockIdx.x * RK, d0 = blockIdx.y * RD;
+    const int k = k0 + kk, d = d0 + dd;
     const long KD = (long)K * D;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;   // i = d * K + k
-    if (i < KD) {
-        const long d = i / K, k = i % K;
-        double s = 0.0;
-        for (int c = 0; c < NC; ++c) s += (double)partRT[(long)c * KD + i];
-        RT[k * D + d] = s;
+    double s = 0.0;
+    if (k < K && d < D) {
+        const float *p = partRT + (long)d * K + k;
+        int c = 0;
+        for (; c + RU <= NC; c += RU) {
+            float v[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) v[u] = p[(long)(c + u) * KD];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) s += (double)v[u];
+        }
+        for (; c < NC; ++c) s += (double)p[(long)c * KD];
     }
-    if (i < K) {
-        double s = 0.0;
-        for (int c = 0; c < NC; ++c) s += (double)partG[(long)c * K + i];
-        G[i] = s;
+    tile[kk][dd] = s;
+    __syncthreads();
+    {
+        const int r = t / RD, e = t % RD;
+        if (k0 + r < K && d0 + e < D) RT[(long)(k0 + r) * D + d0 + e] = tile[r][e];
+    }
+    if (blockIdx.y == 0 && t < RK && k < K) {
+        double g = 0.0;
+        for (int c = 0; c < NC; ++c) g += (double)partG[(long)c * K + k];
+        G[k] = g;
     }
 }
 
@@ -1129,10 +1154,10 @@ int dg_estep_mixed(dg_ctx *c, int isDA, double temp) {
     if (rc) return rc;
     DG_HIP(c, hipGetLastError());
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    const long KD = (long)c->K * c->D;
-    const long nred = KD > c->K ? KD : c->K;
-    hipLaunchKernelGGL(k_reduce32, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, c->stream,
-                       c->partRT32, c->partG32, b->NC, c->K, c->D, c->RT, c->G);
+    hipLaunchKernelGGL(k_reduce32,
+                       dim3((unsigned)((c->K + RK - 1) / RK), (unsigned)((c->D + RD - 1) / RD)),
+                       dim3(RK * RD), 0, c->stream, c->partRT32, c->partG32, b->NC, c->K, c->D,
+                       c->RT, c->G);
     DG_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_sum_partlog, dim3(1), dim3(1024), 0, c->stream, b->partLog, b->Np / PT,
                        c->scal);
