@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["mixed", "mixed_r1"], default="mixed",
+                    help="mixed: R as an FP16 hi/lo pair; mixed_r1: R as one FP16 term")
     ap.add_argument("--batch", action="store_true",
                     help="measure the 64-instance batched workload instead of the headline")
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
@@ -204,7 +206,8 @@ def main():
     T, X, FI, Y0, b0 = build_problem(N, D, K, M, L, seed=7)
 
     # ---- mixed precision (the measured path)
-    e16 = dg.Engine(precision=dg.MIXED, lam=LAMBDA)
+    prec16 = dg.MIXED_R1 if args.precision == "mixed_r1" else dg.MIXED
+    e16 = dg.Engine(precision=prec16, lam=LAMBDA)
     e16.upload(T, FI, Y0, b0)
     e16.set_timing(True)
     print("[bench] mixed warmup", file=sys.stderr, flush=True)
@@ -261,7 +264,9 @@ def main():
         "metric": METRIC, "value": round(it16, 3), "unit": "iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 / it16, 3),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
-        "dtype": "fp16x2-mfma+fp32 (E-step), fp64 (M-step)", "data": "synthetic",
+        "dtype": ("fp16x2-mfma+fp32 (E-step), fp64 (M-step)" if prec16 == dg.MIXED else
+                  "fp16x2 distances, fp16 R, fp32 accumulation (E-step), fp64 (M-step)"),
+        "data": "synthetic",
         "config": {"workload": "N=1,000,000 binary fingerprints D=166 (50 prototypes, "
                                "flip 0.05, seed 7), L=3 K=8000, M+1=126, s=1, lambda=1e-3, "
                                "DA temp 2.0; one step = one E+M iteration",

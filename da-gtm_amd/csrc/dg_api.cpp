@@ -136,7 +136,8 @@ int dg_create(const dg_config *cfg, dg_ctx **out) {
     *out = nullptr;
     dg_config def{0, DG_FP64, 0.0};
     const dg_config &cf = cfg ? *cfg : def;
-    if (cf.precision != DG_FP64 && cf.precision != DG_MIXED) return DG_EINVAL;
+    if (cf.precision != DG_FP64 && cf.precision != DG_MIXED && cf.precision != DG_MIXED_R1)
+        return DG_EINVAL;
     if (cf.lambda < 0.0) return DG_EINVAL;
     dg_ctx *c = new (std::nothrow) dg_ctx();
     if (!c) return DG_ENOMEM;
@@ -221,7 +222,7 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     if ((rc = dg_sum_into(c, c->tt, N, kScalBetaSum)) || (rc = sync_scalars(c))) return rc;
     c->sum_tt = c->h_scal[kScalBetaSum];
     c->loaded = true;
-    if (c->cfg.precision == DG_MIXED) {
+    if (c->mixed()) {
         if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
     }
     DG_HIP(c, hipStreamSynchronize(c->stream));
@@ -242,7 +243,7 @@ int dg_set_state(dg_ctx *c, const double *Y, long ldY, double beta) {
     c->beta = beta;
     c->have_w = false;
     c->have_estep = false;
-    if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
+    if (c->mixed() && (rc = dg_mixed_prepare_y(c))) return rc;
     DG_HIP(c, hipStreamSynchronize(c->stream));
     return DG_OK;
 }
@@ -253,7 +254,7 @@ int dg_estep(dg_ctx *c, int isDA, double temp, dg_estep_out *out) {
     if (isDA && !(temp > 0.0)) return dg_fail(c, DG_EINVAL, "dg_estep: temp must be > 0");
     int rc;
     if ((rc = select_device(c))) return rc;
-    if (c->cfg.precision == DG_MIXED) rc = dg_estep_mixed(c, isDA, temp);
+    if (c->mixed()) rc = dg_estep_mixed(c, isDA, temp);
     else rc = dg_estep_fp64(c, isDA, temp);
     if (rc) return rc;
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[3], c->stream));
@@ -301,7 +302,7 @@ int dg_mstep(dg_ctx *c, double *beta_out) {
     c->have_w = true;
     const double s = c->sum_tt + c->h_scal[kScalBetaSum];
     c->beta = ((double)c->N * c->D) / s;
-    if (c->cfg.precision == DG_MIXED && (rc = dg_mixed_prepare_y(c))) return rc;
+    if (c->mixed() && (rc = dg_mixed_prepare_y(c))) return rc;
     if (beta_out) *beta_out = c->beta;
     return DG_OK;
 }
@@ -393,7 +394,7 @@ int dg_set_basis(dg_ctx *c, const double *FI, int K, int M1, long ldFI, const do
     if ((rc = dg_centres_from_w(c))) return rc;
     // the mixed path's operand layout and chunking depend on K: rebuild it
     // from the resident FP64 data
-    if (c->cfg.precision == DG_MIXED &&
+    if (c->mixed() &&
         ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))))
         return rc;
     DG_HIP(c, hipStreamSynchronize(c->stream));
@@ -410,7 +411,7 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
     int rc;
     if ((rc = select_device(c))) return rc;
     // R in FP64: the mixed path refreshes the FP64 per-point normalisers
-    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+    if (c->mixed() && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
         return rc;
     const int K = c->K, D = c->D;
     const long DD = (long)D * D;
@@ -567,7 +568,7 @@ int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
     }
     // output projections run in FP64; the mixed path refreshes the FP64
     // per-point normalisers from the same model state first
-    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+    if (c->mixed() && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
         return rc;
     double *dX = nullptr, *dM = nullptr;
     if ((rc = dalloc(c, &dX, (size_t)c->K * L))) return rc;
@@ -595,7 +596,7 @@ int dg_resp_dense(dg_ctx *c, double *R, long ldR) {
     if (!R || ldR < c->N) return dg_fail(c, DG_EINVAL, "dg_resp_dense: bad args");
     int rc;
     if ((rc = select_device(c))) return rc;
-    if (c->cfg.precision == DG_MIXED && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
+    if (c->mixed() && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp)))
         return rc;
     double *dR = nullptr;
     if ((rc = dalloc(c, &dR, (size_t)c->K * c->N))) return rc;

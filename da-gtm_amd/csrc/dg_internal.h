@@ -72,6 +72,9 @@ struct dg_ctx {
     float *partRT32 = nullptr;     // NC x D x K
     float *partG32 = nullptr;      // NC x K
 
+    // either mixed-precision E-step
+    bool mixed() const { return cfg.precision != DG_FP64; }
+
     // timing
     int timing = 0;
     hipEvent_t ev[6] = {};

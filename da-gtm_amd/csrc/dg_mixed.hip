@@ -378,7 +378,7 @@ __global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float b
 // that row.  The 1-D grid is mapped so that the workgroups the hardware places
 // on one XCD (linear id mod 8) share a point chunk and walk all its centre
 // groups, keeping that chunk's data in the XCD's L2.
-template <int KS, int TT, bool GM>
+template <int KS, int TT, bool GM, bool RLO>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_e2_mfma(
     const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
@@ -400,7 +400,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     constexpr int OFF_TT = NIA * 512, OFF_P = (NIA + NIB) * 512;
     constexpr int NS = 3;                          // ring slots
     constexpr int NX = (TT + 1) * KS;              // cross-term MFMAs per block
-    constexpr int NR = (TT + 1) * 2 * DB;          // R T MFMAs per block
+    constexpr int RG = TT + (RLO ? 1 : 0);         // R T products per 16 points and d-block
+    constexpr int NR = RG * 2 * DB;                // R T MFMAs per block
     static_assert(NS * SLOT * 2 <= 160 * 1024, "LDS ring");
     __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
 
@@ -513,7 +514,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // R = 2^e as FP16 hi/lo pairs in the B-operand order of the R T product,
     // in 24 steps, three per pair p: the exponentials of elements 2p and
     // 2p+1, then the packs hi = RNE(R) and lo = RNE(R - hi) (v_fma_mix reads
-    // one FP16 half of the packed hi and writes one half of lo).
+    // one FP16 half of the packed hi and writes one half of lo).  Without
+    // RLO, R is the single term hi.
     constexpr int NRS = 24;
     float Rv[16];
     unsigned wh[8], wl[8];
@@ -526,12 +528,14 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         } else {
             const h2 hp = {(_Float16)Rv[2 * p], (_Float16)Rv[2 * p + 1]};
             const unsigned hb = __builtin_bit_cast(unsigned, hp);
-            unsigned lb = 0u;
-            asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "+v"(lb) : "v"(hb), "v"(Rv[2 * p]));
-            asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-                : "+v"(lb) : "v"(hb), "v"(Rv[2 * p + 1]));
             wh[p] = hb;
-            wl[p] = lb;
+            if (RLO) {
+                unsigned lb = 0u;
+                asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "+v"(lb) : "v"(hb), "v"(Rv[2 * p]));
+                asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+                    : "+v"(lb) : "v"(hb), "v"(Rv[2 * p + 1]));
+                wl[p] = lb;
+            }
         }
     };
     auto resp_ops = [&](h8 (&rhi)[2], h8 (&rlo)[2]) {
@@ -539,25 +543,31 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         // inside the asm, so it would not insert the two wait states an MFMA
         // needs before reading a VGPR that a VALU instruction has just
         // written: fence the scheduler and wait here, once per block.
-        __builtin_amdgcn_sched_barrier(0);
-        asm volatile("s_nop 1");
-        __builtin_amdgcn_sched_barrier(0);
+        if (RLO) {
+            __builtin_amdgcn_sched_barrier(0);
+            asm volatile("s_nop 1");
+            __builtin_amdgcn_sched_barrier(0);
+        }
 #pragma unroll
         for (int sh = 0; sh < 2; ++sh) {
             const u4 a = {wh[4 * sh], wh[4 * sh + 1], wh[4 * sh + 2], wh[4 * sh + 3]};
-            const u4 b = {wl[4 * sh], wl[4 * sh + 1], wl[4 * sh + 2], wl[4 * sh + 3]};
             rhi[sh] = __builtin_bit_cast(h8, a);
-            rlo[sh] = __builtin_bit_cast(h8, b);
+            if (RLO) {
+                const u4 b = {wl[4 * sh], wl[4 * sh + 1], wl[4 * sh + 2], wl[4 * sh + 3]};
+                rlo[sh] = __builtin_bit_cast(h8, b);
+            } else {
+                rlo[sh] = rhi[sh];                 // unused
+            }
         }
         if (!GM) { kahan(G, Gc, gs); gs = 0.f; }
     };
     // R T MFMAs: accumulator d-block db, 16-point half sh, term wq (hi R,
-    // lo R, second data term).  Issue order: by sh, then by pairs of d-blocks,
+    // lo R when RLO, then hi R with the second data term).  Issue order: by sh, then by pairs of d-blocks,
     // alternating the two accumulators of a pair so that no MFMA waits for
     // the one just before it.
     struct RtOp { int db, sh, wq, pairFirst; };
     auto rt_op = [&](int i) {
-        constexpr int GP = TT + 1;
+        constexpr int GP = RG;
         const int sh = i / (DB * GP), r = i % (DB * GP);
         const int pair = r / (2 * GP), in = r % (2 * GP);
         const int npd = (2 * pair + 1 < DB) ? 2 : 1;
@@ -566,7 +576,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     auto rmfma = [&](const RtOp &o, const h8 &u0, const h8 &u1, const h8 (&rhi)[2],
                      const h8 (&rlo)[2]) {
         if (o.wq == 0) acc[o.db] = mfma(u0, rhi[o.sh], acc[o.db]);
-        else if (o.wq == 1) acc[o.db] = mfma(u0, rlo[o.sh], acc[o.db]);
+        else if (RLO && o.wq == 1) acc[o.db] = mfma(u0, rlo[o.sh], acc[o.db]);
         else acc[o.db] = mfma(u1, rhi[o.sh], acc[o.db]);
     };
     // whole-block versions (prologue and epilogue)
@@ -669,7 +679,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             for (int i = 0; i < NR; ++i) {
                 const RtOp o = rt_op(i);
                 if (o.pairFirst) {
-                    // the next pair starts after this one's 2 (TT + 1) MFMAs (or fewer)
+                    // the next pair starts after this one's 2 RG MFMAs (or fewer)
                     int nx = i + 1;
                     while (nx < NR && !rt_op(nx).pairFirst) ++nx;
                     if (nx < NR) ld_pair(nx);
@@ -947,6 +957,20 @@ struct MixedBufs {
 
 MixedBufs *mb(dg_ctx *c) { return reinterpret_cast<MixedBufs *>(c->mix); }
 
+template <int KS, int TT, bool GM>
+void launch_e2(dg_ctx *c, MixedBufs *b, float beta, float it, long chunk, int KG) {
+    if (c->cfg.precision == DG_MIXED_R1)
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
+                           0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
+                           c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
+                           chunk, KG, c->partRT32, c->partG32);
+    else
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
+                           0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
+                           c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
+                           chunk, KG, c->partRT32, c->partG32);
+}
+
 template <int KS, int TT>
 void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int isDA,
                   double logK) {
@@ -967,16 +991,8 @@ void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int is
     const int KG = b->Kp / CT;
     // G comes out of the R T product when the padded d range has a spare row
     constexpr int DR = 32 * ((16 * KS + 31) / 32);
-    if (DR > c->D)
-        hipLaunchKernelGGL((k_e2_mfma<KS, TT, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
-                           c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
-                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32);
-    else
-        hipLaunchKernelGGL((k_e2_mfma<KS, TT, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH), 0,
-                           c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32,
-                           c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32);
+    if (DR > c->D) launch_e2<KS, TT, true>(c, b, beta, it, chunk, KG);
+    else launch_e2<KS, TT, false>(c, b, beta, it, chunk, KG);
 }
 
 template <int TT>

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdagtm_gpu.so")
 
 DG_OK, DG_EINVAL, DG_ENOMEM, DG_EHIP, DG_ESTATE, DG_ESOLVE = 0, 1, 2, 3, 4, 5
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_R1 = 0, 1, 2
 SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",

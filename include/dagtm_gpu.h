@@ -34,8 +34,11 @@ extern "C" {
 typedef struct dg_ctx dg_ctx;
 
 enum dg_precision {
-    DG_FP64  = 0,   /* every step in FP64 (the FP64 GPU oracle) */
-    DG_MIXED = 1    /* 16-bit split MFMA distances, FP32 epilogue, FP64 M-step */
+    DG_FP64     = 0,  /* every step in FP64 (the FP64 GPU oracle) */
+    DG_MIXED    = 1,  /* 16-bit split MFMA distances, FP32 epilogue, FP64 M-step */
+    DG_MIXED_R1 = 2   /* DG_MIXED with R carried as one FP16 term in the R T
+                         product (not a hi/lo pair): a quarter fewer pass-2
+                         MFMAs; G and R T to ~2e-5 relative instead of ~1e-8 */
 };
 
 typedef struct {
