@@ -52,6 +52,28 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+// Runs fn(t) for t = 0 .. nth - 1 on up to nth threads; a thread that cannot
+// be started runs its share on the calling thread.  fn must not throw.
+template <class F>
+void run_threads(unsigned nth, F &&fn) {
+    std::vector<std::thread> pool;
+    pool.reserve(nth);
+    unsigned started = 1;
+    try {
+        for (; started < nth; ++started) pool.emplace_back(fn, started);
+    } catch (...) {
+    }
+    fn(0u);
+    for (unsigned t = started; t < nth; ++t) fn(t);
+    for (auto &th : pool) th.join();
+}
+
+// A device buffer freed when it goes out of scope (error and exception paths).
+struct DevBuf {
+    double *p = nullptr;
+    ~DevBuf() { dfree(p); }
+};
+
 // Device state whose size depends on K: basis, centres, per-centre
 // statistics and the partial sums of their reductions.
 void free_kstate(dg_ctx *c) {
@@ -401,9 +423,21 @@ int dg_set_basis(dg_ctx *c, const double *FI, int K, int M1, long ldFI, const do
     return DG_OK;
 }
 
+static int next_ctemp_impl(dg_ctx *c, double temp, double eps, const double *gg,
+                           double *ctemp_max, double *ctemp_k);
+
 int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *ctemp_max,
                   double *ctemp_k) {
     if (!c) return DG_EINVAL;
+    try {
+        return next_ctemp_impl(c, temp, eps, gg, ctemp_max, ctemp_k);
+    } catch (...) {
+        return dg_fail(c, DG_ENOMEM, "dg_next_ctemp: host allocation failed");
+    }
+}
+
+static int next_ctemp_impl(dg_ctx *c, double temp, double eps, const double *gg,
+                           double *ctemp_max, double *ctemp_k) {
     if (!c->loaded || !c->have_estep)
         return dg_fail(c, DG_ESTATE, "dg_next_ctemp needs a dg_estep at the current state");
     if (!ctemp_max || !std::isfinite(temp) || !std::isfinite(eps))
@@ -430,9 +464,10 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
     long nc = (2048 + (long)pairs * KB - 1) / ((long)pairs * KB);
     nc = std::max(1L, std::min({nc, 64L, tiles}));
     const size_t slen = (size_t)KB * 2 * DD;
-    double *dPart = nullptr, *dS = nullptr;
-    if ((rc = dalloc(c, &dPart, slen * nc))) return rc;
-    if (nc > 1 && (rc = dalloc(c, &dS, slen))) { dfree(dPart); return rc; }
+    DevBuf bPart, bS;
+    if ((rc = dalloc(c, &bPart.p, slen * nc))) return rc;
+    if (nc > 1 && (rc = dalloc(c, &bS.p, slen))) return rc;
+    double *dPart = bPart.p, *dS = bS.p;
     std::vector<double> hS(slen), ck(K);
     const double GG_MIN = 1e-250;   // mpicomm.c:617-628
     int bad = 0;
@@ -449,8 +484,15 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
         // per centre: eig(H) of mpicomm.c:599-626 with H = [[2S1 - S2, -S2], [-S2, 2S1 - S2]]
         // is eig(2 S1) together with eig(2 S1 - 2 S2) (H is block-symmetric)
         std::atomic<int> fail{0};
-        auto work = [&](unsigned t) {
-            std::vector<double> A((size_t)DD), w(D);
+        auto work = [&](unsigned t) noexcept {
+            std::vector<double> A, w;
+            try {
+                A.resize((size_t)DD);
+                w.resize(D);
+            } catch (...) {
+                fail.fetch_or(2);
+                return;
+            }
             for (int i = (int)t; i < kb; i += (int)nth) {
                 const int k = k0 + i;
                 if (g[k] == 0.0) { ck[k] = HUGE_VAL; continue; }
@@ -465,7 +507,8 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
                             const double v = part ? s1 - 2.0 * S2[(size_t)a * D + b] : s1;
                             A[(size_t)a * D + b] = v * sc;
                         }
-                    if (dg_sym_eigvals(D, A.data(), D, w.data())) { fail = 1; return; }
+                    const int er = dg_sym_eigvals(D, A.data(), D, w.data());
+                    if (er) { fail.fetch_or(er == 2 ? 2 : 1); return; }
                     for (int j = 0; j < D; ++j) {
                         const double l = (w[j] / sc >= limit) ? 0.0 : w[j] / sc;   // comm.c:2149-2161
                         if (l > lmax) lmax = l;
@@ -474,15 +517,13 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
                 ck[k] = c->beta / 2.0 / g[k] * lmax;
             }
         };
-        std::vector<std::thread> pool;
-        for (unsigned t = 1; t < nth; ++t) pool.emplace_back(work, t);
-        work(0);
-        for (auto &th : pool) th.join();
+        run_threads(nth, work);
         bad |= fail.load();
     }
-    dfree(dPart); dfree(dS);
     if (rc) return rc;
-    if (bad) return dg_fail(c, DG_ESOLVE, "dg_next_ctemp: eigenvalue iteration did not converge");
+    // bit 1: an eigenvalue iteration did not converge; bit 2: host allocation failed
+    if (bad & 2) return dg_fail(c, DG_ENOMEM, "dg_next_ctemp: host allocation failed");
+    if (bad & 1) return dg_fail(c, DG_ESOLVE, "dg_next_ctemp: eigenvalue iteration did not converge");
     double cmax = -HUGE_VAL;
     for (int k = 0; k < K; ++k) cmax = std::max(cmax, ck[k]);
     *ctemp_max = cmax;
@@ -492,8 +533,20 @@ int dg_next_ctemp(dg_ctx *c, double temp, double eps, const double *gg, double *
 
 constexpr int kMaxBatchWorkers = 4;
 
+static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_item *items,
+                             int n, dg_anneal_result *res, int *status, double *phase_ms);
+
 int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *items, int n,
                     dg_anneal_result *res, int *status, double *phase_ms) {
+    try {
+        return batch_anneal_impl(cfg, workers, items, n, res, status, phase_ms);
+    } catch (...) {
+        return DG_ENOMEM;
+    }
+}
+
+static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_item *items,
+                             int n, dg_anneal_result *res, int *status, double *phase_ms) {
     if (!cfg || n < 0 || (n > 0 && (!items || !res || !status))) return DG_EINVAL;
     if (phase_ms) phase_ms[0] = phase_ms[1] = 0.0;
     if (n == 0) return DG_OK;
@@ -518,13 +571,15 @@ int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *item
     std::vector<dg_ctx *> ctx(n, nullptr);
     auto run_pool = [&](const std::function<void(int)> &job) {
         std::atomic<int> next{0};
-        auto worker = [&]() {
-            for (int j; (j = next.fetch_add(1)) < n;) job(order[j]);
-        };
-        std::vector<std::thread> pool;
-        for (int t = 1; t < workers; ++t) pool.emplace_back(worker);
-        worker();
-        for (auto &th : pool) th.join();
+        run_threads((unsigned)workers, [&](unsigned) noexcept {
+            for (int j; (j = next.fetch_add(1)) < n;) {
+                try {
+                    job(order[j]);
+                } catch (...) {
+                    status[order[j]] = DG_ENOMEM;
+                }
+            }
+        });
     };
     using clk = std::chrono::steady_clock;
     // phase 1: one context per instance, data resident on the device

@@ -118,10 +118,18 @@ bool ql_implicit(int n, std::vector<double> &d, std::vector<double> &e) {
 }  // namespace
 
 // Eigenvalues (unordered) of the symmetric n x n matrix A (lower triangle
-// used, row stride lda) into w.  Returns 0, or 1 if the iteration failed.
+// used, row stride lda) into w.  Returns 0, 1 if the iteration failed, or 2
+// if its workspace could not be allocated.
 extern "C" int dg_sym_eigvals(int n, const double *A, long lda, double *w) {
     if (n <= 0) return 0;
-    std::vector<double> M((size_t)n * n), d(n), e(n);
+    std::vector<double> M, d, e;
+    try {
+        M.resize((size_t)n * n);
+        d.resize(n);
+        e.resize(n);
+    } catch (...) {
+        return 2;
+    }
     for (int i = 0; i < n; ++i)
         for (int j = 0; j <= i; ++j) {
             const double a = A[(size_t)i * lda + j];

@@ -34,7 +34,8 @@ struct Opts {
     int L = 3, K = 125, M = 8, maxloop = 0, ckp_every = 100, verbose = 0, device = 0;
     double alpha = 0.99, delta = 0.1, eps = 1e-6, start_temp = 1.0, lambda = 0.0, s = 2.0;
     bool isDA = true, need_ctemp = true, checkpointing = true, log_progress = false;
-    bool save_R = false, mixed = false, init_only = false, interp = false, hda = false;
+    bool save_R = false, init_only = false, interp = false, hda = false;
+    int precision = DG_FP64;
     int Kmax = 32768;
     double eps2 = 1e-3;
     int schedule = DG_SCHED_EXP;
@@ -79,7 +80,7 @@ void usage(const char *prog) {
                  "-W dataset   : dataset name for W (default: W)\n"
                  "-x dataset   : dataset name for X (default: X)\n"
                  "-z PxQ       : accepted for compatibility; the engine uses one GPU\n"
-                 "--precision fp64|mixed   arithmetic of the E-step (default fp64)\n"
+                 "--precision fp64|mixed|mixed_r1  arithmetic of the E-step (default fp64)\n"
                  "--lambda x               M-step ridge lambda/beta (default 0)\n"
                  "--rbf-width s            RBF width in centre spacings (default 2)\n"
                  "--device n               HIP device (default 0)\n"
@@ -154,8 +155,9 @@ int parse(int argc, char **argv, Opts &o) {
             case 'x': o.dX = optarg; break;
             case 'z': break;
             case 1:
-                if (!strcasecmp(optarg, "mixed")) o.mixed = true;
-                else if (!strcasecmp(optarg, "fp64")) o.mixed = false;
+                if (!strcasecmp(optarg, "mixed")) o.precision = DG_MIXED;
+                else if (!strcasecmp(optarg, "mixed_r1")) o.precision = DG_MIXED_R1;
+                else if (!strcasecmp(optarg, "fp64")) o.precision = DG_FP64;
                 else { std::fprintf(stderr, "Unknown precision : %s\n", optarg); return 1; }
                 break;
             case 2: o.lambda = std::atof(optarg); break;
@@ -292,7 +294,7 @@ int main(int argc, char **argv) {
 
     // ---- the engine
     dg_ctx *ctx = nullptr;
-    dg_config cfg{o.device, o.mixed ? DG_MIXED : DG_FP64, o.lambda};
+    dg_config cfg{o.device, o.precision, o.lambda};
     int rc = dg_create(&cfg, &ctx);
     if (rc != DG_OK) {
         std::fprintf(stderr, "dg_create failed (%d): no usable HIP device\n", rc);
