@@ -130,6 +130,22 @@ def batch_bench(args):
     probs = c4_instances()
     n = len(probs)
     prec = dg.MIXED if args.batch_precision == "mixed" else dg.FP64
+    tuned = 0
+    if args.tune and prec != dg.FP64:
+        import tempfile
+        table = os.path.join(tempfile.mkdtemp(), "dg_tune.txt")
+        seen = set()
+        for T, FI, Y0, b0, _ in probs:
+            key = (T.shape[1], FI.shape[0], int(np.ceil(np.log2(T.shape[0]))))
+            if key in seen:
+                continue
+            seen.add(key)
+            with dg.Engine(precision=prec, lam=LAMBDA) as e:
+                e.upload(T, FI, Y0, b0)
+                e.tune(table)
+            tuned += 1
+        print(f"[bench] tuned {tuned} shapes", file=sys.stderr, flush=True)
+        os.environ["DG_TUNE_FILE"] = table
     for iters in (args.warmup, args.steps):
         print(f"[bench] batch run, {iters} iterations per instance", file=sys.stderr, flush=True)
         out, ph = dg.batch_anneal(probs, precision=prec, lam=LAMBDA, workers=args.workers,
@@ -154,6 +170,7 @@ def batch_bench(args):
         "config": {"workload": "64 instances: N log-uniform [1e3, 2e5], D in {12,32,64,166}, "
                                "(L,K,M) in {(2,400,25),(2,1600,25),(3,8000,125)}, seed 2024",
                    "total_N": int(sum(p[0].shape[0] for p in probs)), "workers": args.workers,
+                   "tuned_shapes": tuned,
                    "parallelism": "single GPU, one context and stream per worker"},
         "kd_updates_per_s": knd * args.steps / dt,
         "data_bytes_per_batch_step": tbytes,
@@ -180,6 +197,9 @@ def main():
                     help="measure the 64-instance batched workload instead of the headline")
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--tune", action="store_true",
+                    help="with --batch: tune the pass-2 chunk count of every distinct shape first "
+                         "(dg_tune) and run the batch with that table")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()

@@ -7,9 +7,14 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -142,6 +147,42 @@ double quality(long N, int K, int D, double beta, double sum, double w) {
     return q / (double)N;
 }
 
+// ---- tuned pass-2 chunk counts ------------------------------------------
+// A text table, one line per shape: "<precision> <D> <K> <nbits(N)> <NC>"
+// (the last line for a shape wins), named by the DG_TUNE_FILE environment
+// variable and written by dg_tune.  Shared by every context of the process.
+std::mutex g_tune_mu;
+std::string g_tune_path;
+std::map<std::array<long, 4>, int> g_tune;
+
+std::array<long, 4> tune_key(const dg_ctx *c) {
+    long nb = 0;
+    while (nb < 62 && (1L << nb) < c->N) ++nb;
+    return {(long)c->cfg.precision, (long)c->D, (long)c->K, nb};
+}
+
+// caller holds g_tune_mu
+void tune_load(const std::string &path) {
+    if (path == g_tune_path) return;
+    g_tune.clear();
+    g_tune_path = path;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    long p, d, k, nb, nc;
+    while (std::fscanf(f, "%ld %ld %ld %ld %ld", &p, &d, &k, &nb, &nc) == 5)
+        g_tune[{p, d, k, nb}] = (int)nc;
+    std::fclose(f);
+}
+
+int tune_find(const dg_ctx *c) {
+    const char *path = std::getenv("DG_TUNE_FILE");
+    if (!path || !*path) return 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load(path);
+    auto it = g_tune.find(tune_key(c));
+    return it == g_tune.end() ? 0 : it->second;
+}
+
 }  // namespace
 
 extern "C" {
@@ -245,6 +286,7 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     c->sum_tt = c->h_scal[kScalBetaSum];
     c->loaded = true;
     if (c->mixed()) {
+        c->nc_override = tune_find(c);
         if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
     }
     DG_HIP(c, hipStreamSynchronize(c->stream));
@@ -391,6 +433,53 @@ int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_it
     return DG_OK;
 }
 
+int dg_tune(dg_ctx *c, const char *table_path, int *nc_out) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_tune before dg_upload");
+    if (nc_out) *nc_out = 0;
+    if (!c->mixed()) return DG_OK;             // the FP64 path has no tuned choice
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    long lo = 0, hi = 0;
+    const long heur = dg_mixed_nc_limits(c, &lo, &hi);
+    std::vector<long> cand;
+    for (long v : {heur, heur / 2, heur / 4, heur / 8, heur / 16, lo, 2 * heur}) {
+        v = std::max(lo, std::min(hi, (v + 7) / 8 * 8));
+        if (std::find(cand.begin(), cand.end(), v) == cand.end()) cand.push_back(v);
+    }
+    // time three DA E-steps at temperature 2 (three exponentials per
+    // element) per candidate, after one untimed step
+    using clk = std::chrono::steady_clock;
+    long best = heur;
+    double best_ms = HUGE_VAL;
+    for (long nc : cand) {
+        c->nc_override = (int)nc;
+        if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
+        if ((rc = dg_estep(c, 1, 2.0, nullptr))) return rc;
+        double ms = HUGE_VAL;
+        for (int r = 0; r < 3; ++r) {
+            const auto t0 = clk::now();
+            if ((rc = dg_estep(c, 1, 2.0, nullptr))) return rc;
+            ms = std::min(ms, std::chrono::duration<double, std::milli>(clk::now() - t0).count());
+        }
+        if (ms < best_ms) { best_ms = ms; best = nc; }
+    }
+    c->nc_override = (int)best;
+    if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
+    c->have_estep = false;
+    if (nc_out) *nc_out = (int)best;
+    if (table_path && *table_path) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        FILE *f = std::fopen(table_path, "a");
+        if (!f) return dg_fail(c, DG_EINVAL, std::string("dg_tune: cannot write ") + table_path);
+        const auto k = tune_key(c);
+        std::fprintf(f, "%ld %ld %ld %ld %ld\n", k[0], k[1], k[2], k[3], best);
+        std::fclose(f);
+        if (g_tune_path == table_path) g_tune[k] = (int)best;
+    }
+    return DG_OK;
+}
+
 int dg_set_basis(dg_ctx *c, const double *FI, int K, int M1, long ldFI, const double *W, long ldW) {
     if (!c) return DG_EINVAL;
     if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_set_basis before dg_upload");
@@ -416,6 +505,7 @@ int dg_set_basis(dg_ctx *c, const double *FI, int K, int M1, long ldFI, const do
     if ((rc = dg_centres_from_w(c))) return rc;
     // the mixed path's operand layout and chunking depend on K: rebuild it
     // from the resident FP64 data
+    if (c->mixed()) c->nc_override = tune_find(c);
     if (c->mixed() &&
         ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))))
         return rc;

@@ -72,6 +72,8 @@ struct dg_ctx {
     float *partRT32 = nullptr;     // NC x D x K
     float *partG32 = nullptr;      // NC x K
 
+    int nc_override = 0;           // pass-2 chunk count chosen by the tuner (0: built-in)
+
     // either mixed-precision E-step
     bool mixed() const { return cfg.precision != DG_FP64; }
 
@@ -114,6 +116,11 @@ extern "C" int dg_sym_eigvals(int n, const double *A, long lda, double *w);
 
 // ---- mixed kernels (dg_mixed.hip) ----
 int dg_mixed_prepare_t(dg_ctx *ctx);
+// Built-in pass-2 chunk count for the context's shape; *lo / *hi (may be
+// NULL) receive the smallest count that keeps chunks <= 8192 points and the
+// largest useful one (both multiples of 8).
+long dg_mixed_nc_limits(const dg_ctx *ctx, long *lo, long *hi);
+int dg_mixed_nc(const dg_ctx *ctx);   // chunk count in use
 int dg_mixed_prepare_y(dg_ctx *ctx);
 int dg_estep_mixed(dg_ctx *ctx, int isDA, double temp);
 void dg_mixed_free(dg_ctx *ctx);

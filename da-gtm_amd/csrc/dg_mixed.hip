@@ -1058,6 +1058,31 @@ void dg_mixed_free(dg_ctx *c) {
     c->mixed_ready = 0;
 }
 
+long dg_mixed_nc_limits(const dg_ctx *c, long *lo, long *hi) {
+    const long Np = ((c->N + PT - 1) / PT) * PT;
+    const long Kp = ((c->K + CT - 1) / CT) * CT;
+    const long blocks = Np / BP;
+    // chunks of pass 2: ~8K points each (the FP32 accumulation of R T over a
+    // chunk carries a small systematic bias that grows with its length; at 16K
+    // points the annealing dynamics amplified it to 3e-6 in llh within four
+    // iterations), enough workgroups to fill the chip
+    long nmin = (c->N + 8191) / 8192;
+    long nc = nmin;
+    const long want = (2048 + (Kp / CT) - 1) / (Kp / CT);
+    if (nc < want) nc = want;
+    if (nc > blocks) nc = blocks;
+    if (nc < 1) nc = 1;
+    const long top = ((blocks + 7) / 8) * 8;
+    if (lo) *lo = std::min(((nmin + 7) / 8) * 8, top);
+    if (hi) *hi = top;
+    return ((nc + 7) / 8) * 8;          // the pass-2 grid maps 8 chunks per XCD round
+}
+
+int dg_mixed_nc(const dg_ctx *c) {
+    const MixedBufs *b = reinterpret_cast<const MixedBufs *>(c->mix);
+    return b ? b->NC : 0;
+}
+
 int dg_mixed_prepare_t(dg_ctx *c) {
     dg_mixed_free(c);
     const int KS = ks_for(c->D);
@@ -1071,17 +1096,11 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     c->ldh = ld16;
     b->Np = ((c->N + PT - 1) / PT) * PT;
     b->Kp = ((c->K + CT - 1) / CT) * CT;
-    // chunks of pass 2: ~8K points each (the FP32 accumulation of R T over a
-    // chunk carries a small systematic bias that grows with its length; at 16K
-    // points the annealing dynamics amplified it to 3e-6 in llh within four
-    // iterations), enough workgroups to fill the chip
-    const long blocks = b->Np / BP;
-    long nc = (c->N + 8191) / 8192;
-    const long want = (2048 + (b->Kp / CT) - 1) / (b->Kp / CT);
-    if (nc < want) nc = want;
-    if (nc > blocks) nc = blocks;
-    if (nc < 1) nc = 1;
-    nc = ((nc + 7) / 8) * 8;          // the pass-2 grid maps 8 chunks per XCD round
+    // pass-2 chunk count: the context's override (a tuned value) within the
+    // accuracy and grid limits, else the built-in choice
+    long lo = 0, hi = 0;
+    long nc = dg_mixed_nc_limits(c, &lo, &hi);
+    if (c->nc_override > 0) nc = std::max(lo, std::min(hi, ((long)c->nc_override + 7) / 8 * 8));
     b->NC = (int)nc;
     // chunk length: a multiple of 64 points (transpose tiles never straddle chunks)
     b->CL = ((b->Np + nc * 64 - 1) / (nc * 64)) * 64;

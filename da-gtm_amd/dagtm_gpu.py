@@ -21,7 +21,7 @@ SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
-           "dg_get_timing", "dg_batch_anneal"]
+           "dg_get_timing", "dg_batch_anneal", "dg_tune"]
 
 
 class DgConfig(C.Structure):
@@ -83,6 +83,7 @@ def load():
                             C.c_long, _dp, C.c_long, C.c_double]
     L.dg_set_state.argtypes = [vp, _dp, C.c_long, C.c_double]
     L.dg_set_basis.argtypes = [vp, _dp, C.c_int, C.c_int, C.c_long, vp, C.c_long]
+    L.dg_tune.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
     L.dg_estep.argtypes = [vp, C.c_int, C.c_double, C.POINTER(DgEstepOut)]
     L.dg_mstep.argtypes = [vp, C.POINTER(C.c_double)]
     L.dg_anneal.argtypes = [vp, C.POINTER(DgAnnealParams), C.POINTER(DgAnnealResult),
@@ -161,6 +162,14 @@ class Engine:
         self._chk(self.L.dg_set_basis(self.h, FI, K, M1, M1, None if w is None else w.ctypes.data,
                                       self.D))
         self.K = K
+
+    def tune(self, table_path=None):
+        """Time pass-2 chunk counts for this problem (mixed precision), keep the
+        fastest and optionally append it to a table file; returns the count."""
+        nc = C.c_int()
+        self._chk(self.L.dg_tune(self.h, None if table_path is None else
+                                 os.fsencode(table_path), C.byref(nc)))
+        return nc.value
 
     def estep(self, isDA=True, temp=1.0):
         o = DgEstepOut()

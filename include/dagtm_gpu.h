@@ -186,6 +186,16 @@ int dg_map(dg_ctx *ctx, const double *X, int L, long ldX, double *M,
  * problems (K x N doubles are allocated on the device for the call). */
 int dg_resp_dense(dg_ctx *ctx, double *R, long ldR);
 
+/* Pass-2 chunk count tuning (no counterpart in the reference).  Times the
+ * mixed E-step of the uploaded problem for a few chunk counts within the
+ * accuracy limit (chunks of at most 8192 points), keeps the fastest for this
+ * context, stores it in *nc_out (0 in DG_FP64, which has no tuned choice)
+ * and, when table_path is not NULL, appends "<precision> <D> <K> <nbits(N)>
+ * <count>" to that file.  dg_upload and dg_set_basis read the file named by
+ * the DG_TUNE_FILE environment variable and use its entry for their shape.
+ * Leaves no E-step behind (a dg_estep is needed before dg_mstep). */
+int dg_tune(dg_ctx *ctx, const char *table_path, int *nc_out);
+
 /* Copies the model back.  Replaces the h5save_byrow of W, Y and beta
  * (dagtm.cxx:2420-2480).  Any pointer may be NULL. */
 int dg_download(dg_ctx *ctx, double *W, long ldW, double *Y, long ldY,

@@ -64,3 +64,30 @@ def test_batch_item_errors_are_per_item():
     out, _ = dg.batch_anneal(probs, precision=dg.FP64, workers=2, alpha=0.9, eps=1e-6)
     assert [o[0] for o in out] == [dg.DG_OK, dg.DG_EINVAL, dg.DG_OK, dg.DG_OK]
     assert all(o[1].nloop > 0 for i, o in enumerate(out) if i != 1)
+
+
+def test_tuned_chunk_count_is_used_from_the_table(tmp_path, monkeypatch):
+    """dg_tune keeps the fastest pass-2 chunk count and records it; a context
+    uploaded with DG_TUNE_FILE naming that table uses the same count, so its
+    E-step is bit-identical to the tuned context's."""
+    _say("case: pass-2 chunk tuning table")
+    T, FI, Y0, b0, _ = _instances()[2]
+    table = str(tmp_path / "tune.txt")
+    with dg.Engine(precision=dg.MIXED) as e:
+        e.upload(T, FI, Y0, b0)
+        nc = e.tune(table)
+        ref = e.estep(True, 2.0)
+    line = open(table).read().split()
+    assert [int(x) for x in line] == [dg.MIXED, T.shape[1], FI.shape[0],
+                                      int(np.ceil(np.log2(T.shape[0]))), nc]
+    assert nc > 0 and nc % 8 == 0
+    monkeypatch.setenv("DG_TUNE_FILE", table)
+    with dg.Engine(precision=dg.MIXED) as e:
+        e.upload(T, FI, Y0, b0)
+        got = e.estep(True, 2.0)
+    assert got == ref
+    with dg.Engine(precision=dg.FP64) as e:       # no tuned choice in FP64
+        e.upload(T, FI, Y0, b0)
+        assert e.tune(None) == 0
+        l64 = e.estep(True, 2.0)[0]
+    assert got[0] == pytest.approx(l64, rel=1e-5)
