@@ -69,6 +69,20 @@ __global__ __launch_bounds__(1024) void k_count_zero(const double *__restrict__ 
     if (threadIdx.x == 0) scal[kScalNZero] = s[0];
 }
 
+// tr(A0) into *out, summed by the first wave; every thread of the workgroup
+// calls it (it ends with a barrier).  Only the jitter retries need it.
+__device__ void block_trace(const double *__restrict__ A0, int n, double *out) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        double s = 0.0;
+        for (int i = tid; i < n; i += 64) s += A0[(long)i * n + i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (tid == 0) *out = s;
+    }
+    __syncthreads();
+}
+
 // Cholesky solve of (A0 + (ridge + jitter) I) W = B in one workgroup, the
 // factor built in place in A (right-looking; each entry receives its
 // updates in the same column order as a left-looking loop).  On a
@@ -83,17 +97,15 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double *__restrict__ 
     __shared__ int fail;
     __shared__ double tr;
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += A0[(long)i * n + i];
-        tr = s;
-    }
-    __syncthreads();
     double jit = 0.0;
     int ok = 0;
     for (int attempt = 0; attempt < 9 && !ok; ++attempt) {
-        if (attempt == 1) jit = 1e-13 * tr / n;
-        else if (attempt > 1) jit *= 10.0;
+        if (attempt == 1) {
+            block_trace(A0, n, &tr);
+            jit = 1e-13 * tr / n;
+        } else if (attempt > 1) {
+            jit *= 10.0;
+        }
         for (long e = tid; e < (long)n * n; e += nt) {
             const long i = e / n, j = e % n;
             A[e] = A0[e] + ((i == j) ? ridge + jit : 0.0);
@@ -225,61 +237,128 @@ __global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
 constexpr int kLdsN = 126;   // largest M+1 whose factor fits in LDS
 static_assert(kLdsN <= 128, "the Cholesky and solve kernels give each thread at most 4 rows");
 
+// Broadcast lane l's value of x (l wave-uniform) to the whole wavefront.
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+constexpr int kPB = 16;       // panel width of the blocked Cholesky
+
 // Cholesky of A0 + (ridge + jitter) I with the matrix resident in LDS (one
-// workgroup, n <= 126); same pivots and retries as k_chol_solve.  One barrier
-// per column: step j updates the trailing lower triangle with the column
-// still unscaled, A[i][k] -= A[i][j] A[k][j] / p_j, and scales column j - 1,
-// which no step reads any more.  Writes the factor's lower triangle to L.
+// workgroup, n <= 126); same pivots and retries as k_chol_solve.  Blocked,
+// right-looking: wave 0 factors each 16-column panel in registers (lane r
+// holds rows p0 + r and p0 + r + 64; the pivot row and the L entries of later
+// panel columns are broadcast with readlane, so the panel needs no barrier),
+// then all waves apply the panel to the trailing lower triangle with 4 x 4
+// register tiles.  Two barriers per panel.  Writes the factor's lower
+// triangle (the rest of L is left unspecified).
 __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
                                                    double *__restrict__ L, int n, double ridge,
                                                    double *__restrict__ scal) {
     __shared__ double As[kLdsN * kLdsN];
+    __shared__ double Pt[kPB][128];                  // the factored panel, transposed
     __shared__ double tr;
+    __shared__ int bad;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int tx = tid & 31, ty = tid >> 5;          // 32 x 32 threads over (row, column)
-    if (tid == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += A0[(long)i * n + i];
-        tr = s;
-    }
-    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    const int tx = tid & 31, ty = tid >> 5;
     double jit = 0.0;
     int ok = 0;
     for (int attempt = 0; attempt < 9 && !ok; ++attempt) {
-        if (attempt == 1) jit = 1e-13 * tr / n;
-        else if (attempt > 1) jit *= 10.0;
+        if (attempt == 1) {
+            block_trace(A0, n, &tr);
+            jit = 1e-13 * tr / n;
+        } else if (attempt > 1) {
+            jit *= 10.0;
+        }
         for (int e = tid; e < n * n; e += nt) {
             const int i = e / n, j = e % n;
             As[e] = A0[e] + ((i == j) ? ridge + jit : 0.0);
         }
+        if (tid == 0) bad = 0;
         __syncthreads();
-        bool failed = false;
-        double pprev = 0.0;
-        for (int j = 0; j < n; ++j) {
-            const double p = As[j * n + j];           // unscaled pivot
-            if (!(p > 0.0)) { failed = true; break; } // the same for every thread
-            if (j > 0) {                              // scale column j - 1
-                const double sq = sqrt(pprev);
-                const double inv = 1.0 / sq;
-                const int i = j - 1 + tid;
-                if (i == j - 1) As[(j - 1) * n + (j - 1)] = sq;
-                else if (i < n) As[i * n + (j - 1)] *= inv;
+        for (int p0 = 0; p0 < n; p0 += kPB) {
+            const int pn = min(kPB, n - p0);
+            if (wv == 0) {
+                const int r0 = p0 + lane, r1 = r0 + 64;
+                double a0[kPB], a1[kPB];
+#pragma unroll
+                for (int c = 0; c < kPB; ++c) {
+                    a0[c] = (c < pn && r0 < n) ? As[r0 * n + p0 + c] : 0.0;
+                    a1[c] = (c < pn && r1 < n) ? As[r1 * n + p0 + c] : 0.0;
+                }
+                bool failed = false;
+#pragma unroll
+                for (int j = 0; j < kPB; ++j) {
+                    if (j < pn && !failed) {
+                        const double piv = lane_bcast(a0[j], j);     // row p0 + j is lane j's
+                        if (!(piv > 0.0)) {
+                            failed = true;                           // the same in every lane
+                        } else {
+                            const double sq = sqrt(piv), inv = 1.0 / sq;
+                            if (lane == j) a0[j] = sq;
+                            else if (lane > j) a0[j] *= inv;
+                            a1[j] *= inv;
+#pragma unroll
+                            for (int c = j + 1; c < kPB; ++c) {
+                                if (c < pn) {
+                                    const double lcj = lane_bcast(a0[j], c);   // L[p0 + c][p0 + j]
+                                    if (lane > j) a0[c] = fma(-a0[j], lcj, a0[c]);
+                                    a1[c] = fma(-a1[j], lcj, a1[c]);
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < kPB; ++c)
+                    if (c < pn) {
+                        if (r0 < n) { As[r0 * n + p0 + c] = a0[c]; Pt[c][r0 - p0] = a0[c]; }
+                        if (r1 < n) { As[r1 * n + p0 + c] = a1[c]; Pt[c][r1 - p0] = a1[c]; }
+                    }
+                if (failed && lane == 0) bad = 1;
             }
-            const double rp = 1.0 / p;
-            for (int i = j + 1 + ty; i < n; i += 32) {
-                const double aij = As[i * n + j] * rp;
-                for (int k = j + 1 + tx; k <= i; k += 32)
-                    As[i * n + k] = fma(-aij, As[k * n + j], As[i * n + k]);
+            __syncthreads();
+            if (bad) break;                                  // read after the barrier: uniform
+            // trailing update A[i][k] -= sum_j L[i][p0 + j] L[k][p0 + j] for
+            // q0 <= k <= i: thread (ty, tx) owns rows q0 + ty + 32 a and columns
+            // q0 + tx + 32 b, read from the transposed panel (consecutive lanes,
+            // consecutive columns: no bank conflicts)
+            const int q0 = p0 + pn;
+            if (q0 + ty < n) {
+                double acc[4][4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+                const int oi = q0 - p0 + ty, ok0 = q0 - p0 + tx;   // panel-relative rows
+                for (int j = 0; j < pn; ++j) {
+                    double li[4], lk[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        li[a] = (oi + 32 * a < 128) ? Pt[j][oi + 32 * a] : 0.0;
+                        lk[a] = (ok0 + 32 * a < 128) ? Pt[j][ok0 + 32 * a] : 0.0;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) acc[a][b] = fma(li[a], lk[b], acc[a][b]);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int i = q0 + ty + 32 * a, k = q0 + tx + 32 * b;
+                        if (i < n && k <= i) As[i * n + k] -= acc[a][b];
+                    }
             }
-            pprev = p;
             __syncthreads();
         }
-        if (!failed) {                                // scale the last column
-            if (tid == 0) As[(n - 1) * n + (n - 1)] = sqrt(pprev);
-            __syncthreads();
-        }
-        ok = !failed;
-        __syncthreads();
+        ok = !bad;
+        __syncthreads();                                     // before the next attempt resets bad
     }
     if (tid == 0) {
         scal[kScalJitter] = jit;
