@@ -214,8 +214,9 @@ def main():
         print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; the N-sharded multi-GPU design is "
-              "described in DESIGN.md but not built, so --gpus must be 1", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; sharding N over several GPUs (one "
+              "all-reduce of G and R T per iteration) is designed but not built, so --gpus "
+              "must be 1", file=sys.stderr)
         sys.exit(2)
     if args.batch:
         batch_bench(args)
