@@ -229,7 +229,9 @@ def main():
     # ---- mixed precision (the measured path)
     prec16 = dg.MIXED_R1 if args.precision == "mixed_r1" else dg.MIXED
     e16 = dg.Engine(precision=prec16, lam=LAMBDA)
-    e16.upload(T, FI, Y0, b0)
+    t_up = time.perf_counter()
+    e16.upload(T, FI, Y0, b0)            # host buffers cross PCIe once, here
+    upload_s = time.perf_counter() - t_up
     e16.set_timing(True)
     print("[bench] mixed warmup", file=sys.stderr, flush=True)
     # the state after the first iteration is compared with the FP64 path's
@@ -298,6 +300,7 @@ def main():
         "state_rel_err_vs_fp64_after_1_iter": state_err,
         "kd_updates_per_s": K * N * D * it16,
         "phase_ms_mixed": [round(x / args.steps, 4) for x in ph16],
+        "upload_s": round(upload_s, 3),
         "phase_ms_fp64": [round(x / args.fp64_steps, 4) for x in ph64],
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -269,10 +269,21 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
         free_problem(c);
         return rc;
     }
-    DG_HIP(c, hipMemsetAsync(c->T64, 0, (size_t)N * ld * sizeof(double), c->stream));
-    DG_HIP(c, hipMemcpy2DAsync(c->T64, ld * sizeof(double), T, (size_t)ldT * sizeof(double),
-                               (size_t)D * sizeof(double), (size_t)N, hipMemcpyHostToDevice,
-                               c->stream));
+    if (ldT == D && (size_t)D != ld) {
+        // dense host rows: one contiguous copy (a strided copy of N short rows
+        // from pageable memory is far slower), then padded on the device
+        DevBuf tmp;
+        if ((rc = dalloc(c, &tmp.p, (size_t)N * D))) { free_problem(c); return rc; }
+        DG_HIP(c, hipMemcpyAsync(tmp.p, T, (size_t)N * D * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        if ((rc = dg_pad_rows(c, tmp.p, N, D, (int)ld, c->T64))) return rc;
+        DG_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        DG_HIP(c, hipMemsetAsync(c->T64, 0, (size_t)N * ld * sizeof(double), c->stream));
+        DG_HIP(c, hipMemcpy2DAsync(c->T64, ld * sizeof(double), T, (size_t)ldT * sizeof(double),
+                                   (size_t)D * sizeof(double), (size_t)N, hipMemcpyHostToDevice,
+                                   c->stream));
+    }
     DG_HIP(c, hipMemcpy2DAsync(c->FI, (size_t)M1 * sizeof(double), FI,
                                (size_t)ldFI * sizeof(double), (size_t)M1 * sizeof(double),
                                (size_t)K, hipMemcpyHostToDevice, c->stream));

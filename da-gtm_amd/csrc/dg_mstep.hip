@@ -530,6 +530,24 @@ __global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ p
 
 }  // namespace
 
+// dst[r * ld + d] = src[r * D + d] for d < D, 0 for D <= d < ld.
+__global__ __launch_bounds__(256) void k_pad_rows(const double *__restrict__ src, long rows,
+                                                  int D, int ld, double *__restrict__ dst) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * ld) return;
+    const long r = e / ld;
+    const int d = (int)(e % ld);
+    dst[e] = (d < D) ? src[r * D + d] : 0.0;
+}
+
+int dg_pad_rows(dg_ctx *c, const double *src, long rows, int D, int ld, double *dst) {
+    const long n = rows * ld;
+    hipLaunchKernelGGL(k_pad_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src,
+                       rows, D, ld, dst);
+    DG_HIP(c, hipGetLastError());
+    return DG_OK;
+}
+
 int dg_rownorms(dg_ctx *c, const double *X, long rows, int ld, int D, double *out) {
     hipLaunchKernelGGL(k_rownorm, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, c->stream,
                        X, rows, ld, D, out);
