@@ -56,6 +56,16 @@ __device__ __forceinline__ f16v mfma(h8 a, h8 b, f16v c) {
 
 __device__ __forceinline__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// max of x over lanes l and l ^ 32.  v_permlane32_swap exchanges the upper
+// half of its first operand with the lower half of its second; with x in both,
+// every lane ends up holding {x_l, x_(l^32)} in the two results.  A VALU
+// instruction, unlike ds_bpermute, which goes through the LDS pipeline.
+__device__ __forceinline__ float xhalf_max(float x) {
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return fmaxf(__builtin_bit_cast(float, (unsigned)p[0]), __builtin_bit_cast(float, (unsigned)p[1]));
+}
+
 __device__ __forceinline__ void kahan(float &s, float &c, float x) {
     const float y = x - c;
     const float t = s + y;
@@ -75,9 +85,12 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 // exponentials and sums of one centre row of tile t (or a DMA load of tile
 // t+2); tile t+1's exponents, maximum and rescaling then follow in the basic
 // block of its MFMAs.  Only VALU results (the exponents) cross the loop's back
-// edge.
+// edge.  The register cap of two waves per SIMD (the kernel needs at most 146
+// registers and runs three) makes the compiler keep the accumulator in
+// ordinary VGPRs, where the exponent FMAs read it directly; in AGPRs every
+// element would first need a v_accvgpr_read.
 template <int KS, int TT, bool TWO>
-__global__ __launch_bounds__(NTH) void k_e1_mfma(
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k_e1_mfma(
     const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
@@ -178,7 +191,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
                 zt = fmaxf(zt, z[4 * m4 + q]);
             }
         }
-        zt = fmaxf(zt, __shfl_xor(zt, 32));
+        zt = xhalf_max(zt);
         // The underflow mask only matters when a masked element could weigh
         // more than 2^-64 of the running sum, i.e. when the tile maximum is
         // within kMaskMargin * temp of the threshold; otherwise skip it.
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
                 z[r] = (z[r] < thr) ? -INFINITY : z[r];
                 zt = fmaxf(zt, z[r]);
             }
-            zt = fmaxf(zt, __shfl_xor(zt, 32));
+            zt = xhalf_max(zt);
         }
         if (zt > m) {
             const float dm = m - zt;
@@ -204,12 +217,18 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         nmit = -m * it;
     };
     // exponentials and sums of a tile, in steps: step 2r adds centre row r to
-    // the first sum, step 2r+1 to the second (nothing when !TWO)
+    // the first sum, step 2r+1 to the second (nothing when !TWO); row 0
+    // starts the sums (an add to 0.0f is not folded away: -0 + 0 = +0)
     constexpr int NRW = 32;
     auto row_sum = [&](const float (&z)[16], int st) {
         const int r = st >> 1;
-        if ((st & 1) == 0) t1 += ex2(z[r] - m);
-        else if (TWO) t2 += ex2(fmaf(z[r], it, nmit));
+        if ((st & 1) == 0) {
+            const float v = ex2(z[r] - m);
+            t1 = (r == 0) ? v : t1 + v;
+        } else if (TWO) {
+            const float v = ex2(fmaf(z[r], it, nmit));
+            t2 = (r == 0) ? v : t2 + v;
+        }
     };
     // The empty asm pins the sums at this point: without it the compiler
     // sinks the whole chain of exponentials below the next branch, out of the
@@ -218,7 +237,6 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         kahan(S1, C1, t1);
         if (TWO) kahan(S2, C2, t2);
         asm volatile("" : "+v"(S1), "+v"(C1), "+v"(S2), "+v"(C2));
-        t1 = 0.f; t2 = 0.f;
     };
 
 #pragma unroll
@@ -238,6 +256,8 @@ __global__ __launch_bounds__(NTH) void k_e1_mfma(
         }
         tile_exp(0, c, z);
     }
+    // iteration 0 refills tile 0's slot: every wave must be done reading it
+    __syncthreads();
     // work steps spread evenly over the MFMA slots: the row sums of tile t,
     // then the DMA loads (slot m runs steps m W1 / NX .. (m + 1) W1 / NX - 1)
     constexpr int W1 = NRW + NIW;

@@ -96,3 +96,20 @@ def test_checker_sees_an_unfenced_read():
             "v_mfma_f32_32x32x16_f16 a[0:15], v[20:23], v[0:3], a[0:15]"]
     assert _min_mix_to_mfma(body) == 1
     assert _min_mix_to_mfma(body[:2] + ["s_nop 1", "s_branch .LBB0_1", ".LBB0_1:"] + body[2:]) is None
+
+
+def test_pass1_accumulator_in_vgprs(asm):
+    """Pass 1 reads its accumulator with VALU FMAs: it must live in ordinary
+    VGPRs (no AGPRs, no v_accvgpr_read), and every instantiation must still fit
+    three waves per SIMD (at most 168 registers)."""
+    text = "\n".join(asm)
+    names = sorted(set(re.findall(r"^(_Z\w*k_e1_mfma\w*):", text, re.M)))
+    assert len(names) >= 16, "pass-1 instantiations not found"
+    for name in names:
+        agpr = re.search(r"\.set " + name + r"\.num_agpr, (\d+)", text)
+        vgpr = re.search(r"\.set " + name + r"\.num_vgpr, (\d+)", text)
+        assert agpr and vgpr, name
+        assert int(agpr.group(1)) == 0, (name, agpr.group(1))
+        assert int(vgpr.group(1)) <= 168, (name, vgpr.group(1))
+    for name, body in _kernels(asm, "k_e1_mfma").items():
+        assert not any(x.startswith("v_accvgpr_read") for x in body), name
