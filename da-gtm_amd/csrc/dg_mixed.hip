@@ -161,6 +161,16 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
             (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
             (__attribute__((address_space(3))) void *)&smem[s][I * 512], 16, 0, 0);
     };
+    // Marks the lane offsets as redefined once per loop iteration (an empty
+    // asm, no instruction).  Otherwise the compiler hoists base + offset into
+    // a 64-bit VGPR pair per load and adds the block step with a VALU
+    // v_lshl_add_u64 before every DMA; with the offsets opaque, the address
+    // stays a scalar base (SGPR pair, SALU arithmetic) plus a 32-bit lane
+    // offset, the saddr form of global_load_lds.
+    auto dma_offsets_opaque = [&]() {
+#pragma unroll
+        for (int i = 0; i < NIW; ++i) asm volatile("" : "+v"(voff[i]));
+    };
     auto ldY = [&](int s, int t, int ks) {         // A operand: centre row j, term t
         return *(const h8 *)&smem[s][(t * KT + j) * LR + 16 * ks + 8 * h];
     };
@@ -262,6 +272,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     // then the DMA loads (slot m runs steps m W1 / NX .. (m + 1) W1 / NX - 1)
     constexpr int W1 = NRW + NIW;
     for (int t = 0; t + 1 < ntiles; ++t) {
+        dma_offsets_opaque();
         const int s1 = (t + 1) % NS, s2 = (t + 2) % NS;
         f16v c;
 #pragma unroll
@@ -497,6 +508,16 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
             (__attribute__((address_space(3))) void *)&smem[s][I * 512], 16, 0, 0);
     };
+    // Marks the lane offsets as redefined once per loop iteration (an empty
+    // asm, no instruction).  Otherwise the compiler hoists base + offset into
+    // a 64-bit VGPR pair per load and adds the block step with a VALU
+    // v_lshl_add_u64 before every DMA; with the offsets opaque, the address
+    // stays a scalar base (SGPR pair, SALU arithmetic) plus a 32-bit lane
+    // offset, the saddr form of global_load_lds.
+    auto dma_offsets_opaque = [&]() {
+#pragma unroll
+        for (int i = 0; i < NIW; ++i) asm volatile("" : "+v"(voff[i]));
+    };
     // A operands: data rows (cross term) and transposed rows (R T)
     auto ldT = [&](int s, int t, int ks) {
         return *(const h8 *)&smem[s][(t * BP + j) * LR + 16 * ks + 8 * h];
@@ -651,6 +672,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         constexpr int W1 = NRS + NIW;              // phase 1: R steps, DMA loads
         constexpr int W2 = 4 + 16;                 // phase 2: pB loads, exponents
         for (int b = 0; b + 1 < nblocks; ++b) {
+            dma_offsets_opaque();
             const int s0 = b % NS, s1 = (b + 1) % NS, s2 = (b + 2) % NS;
             f16v c;
 #pragma unroll

@@ -113,3 +113,17 @@ def test_pass1_accumulator_in_vgprs(asm):
         assert int(vgpr.group(1)) <= 168, (name, vgpr.group(1))
     for name, body in _kernels(asm, "k_e1_mfma").items():
         assert not any(x.startswith("v_accvgpr_read") for x in body), name
+
+
+def test_dma_uses_scalar_base(asm):
+    """Every LDS-DMA of the E-pass streams takes its address as a scalar base
+    plus a 32-bit lane offset (global_load_lds ... vN, s[a:b]): no per-load
+    64-bit VGPR address, which would cost a VALU add per load."""
+    for stem in ("k_e1_mfma", "k_e2_mfma"):
+        ks = _kernels(asm, stem)
+        assert ks, stem
+        for name, body in ks.items():
+            loads = [x for x in body if x.startswith("global_load_lds")]
+            assert loads, name
+            bad = [x for x in loads if not re.match(r"global_load_lds_dwordx4 v\d+, s\[\d+:\d+\]", x)]
+            assert not bad, (name, bad[:2])
