@@ -571,8 +571,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             const unsigned hb = __builtin_bit_cast(unsigned, hp);
             wh[p] = hb;
             if (RLO) {
-                unsigned lb = 0u;
-                asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "+v"(lb) : "v"(hb), "v"(Rv[2 * p]));
+                // mixlo writes the low half and mixhi the high half of lb, so
+                // lb needs no initial value (early clobber: no input shares it)
+                unsigned lb;
+                asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=&v"(lb) : "v"(hb), "v"(Rv[2 * p]));
                 asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
                     : "+v"(lb) : "v"(hb), "v"(Rv[2 * p + 1]));
                 wl[p] = lb;
@@ -671,9 +673,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         // with W steps runs steps m W / n .. (m + 1) W / n - 1
         constexpr int W1 = NRS + NIW;              // phase 1: R steps, DMA loads
         constexpr int W2 = 4 + 16;                 // phase 2: pB loads, exponents
-        for (int b = 0; b + 1 < nblocks; ++b) {
+        // One iteration per block b; the ring slots are template constants, so
+        // LDS addresses fold into the instructions' offset fields (the loop
+        // below runs three iterations per trip, one per slot rotation).
+        auto iter = [&](const int b, auto S0) {
+            constexpr int s0 = decltype(S0)::value, s1 = (s0 + 1) % NS, s2 = (s0 + 2) % NS;
             dma_offsets_opaque();
-            const int s0 = b % NS, s1 = (b + 1) % NS, s2 = (b + 2) % NS;
             f16v c;
 #pragma unroll
             for (int r = 0; r < 16; ++r) c[r] = 0.f;
@@ -740,6 +745,14 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
             for (int r = 0; r < 16; ++r) e[r] = en[r];
             __syncthreads();                       // also waits for this iteration's DMA
+        };
+        static_assert(NS == 3, "the loop unrolls one trip per slot rotation");
+        for (int b = 0; b + 1 < nblocks; b += 3) {
+            iter(b, std::integral_constant<int, 0>{});
+            if (b + 2 >= nblocks) break;
+            iter(b + 1, std::integral_constant<int, 1>{});
+            if (b + 3 >= nblocks) break;
+            iter(b + 2, std::integral_constant<int, 2>{});
         }
         resp_rt((nblocks - 1) % NS, e);
     }
