@@ -740,8 +740,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // rarely taken; a real branch after every MFMA of the iteration
-            if (blkThr[blk0 + b + 1] > -kMaskMargin) mask(s1, en);
+            // rarely taken; a real branch after every MFMA of the iteration.  The
+            // index goes through readfirstlane so that the load stays a scalar
+            // one: in the unrolled loop the compiler otherwise makes it a vector
+            // load whose vmcnt(0) wait exposes a memory round trip per block.
+            if (blkThr[blk0 + __builtin_amdgcn_readfirstlane(b + 1)] > -kMaskMargin) mask(s1, en);
 #pragma unroll
             for (int r = 0; r < 16; ++r) e[r] = en[r];
             __syncthreads();                       // also waits for this iteration's DMA

@@ -127,3 +127,11 @@ def test_dma_uses_scalar_base(asm):
             assert loads, name
             bad = [x for x in loads if not re.match(r"global_load_lds_dwordx4 v\d+, s\[\d+:\d+\]", x)]
             assert not bad, (name, bad[:2])
+
+
+def test_pass2_loop_has_no_vector_scalar_loads(asm):
+    """The per-block mask-threshold read of pass 2 is a scalar load: at most
+    the prologue's single global_load_dword remains in each instantiation."""
+    for name, body in _kernels(asm, "k_e2_mfma").items():
+        n = sum(1 for x in body if x.startswith("global_load_dword "))
+        assert n <= 1, (name, n)
