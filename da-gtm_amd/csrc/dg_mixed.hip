@@ -688,6 +688,21 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 xa0[ks] = ldT(s1, 0, ks);
                 if (TT == 2) xa1[ks] = ldT(s1, 1, ks);
             }
+            // R T A operands (block b's transposed rows, resident all iteration),
+            // indexed g = 2 db + sh: the first pair's are read three MFMAs
+            // before phase 1 ends, each pair's first MFMA reads the next pair's
+            h8 ru0[2 * DB], ru1[2 * DB];
+            auto ld_pair = [&](int i0) {           // operands of the pair starting at MFMA i0
+                const RtOp o = rt_op(i0);
+                const int npd = (o.db + 1 < DB && rt_op(i0 + 1).db != o.db) ? 2 : 1;
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    if (x < npd) {
+                        const int g = 2 * (o.db + x) + o.sh;
+                        ru0[g] = ldTt(s0, 0, g);
+                        if (TT == 2) ru1[g] = ldTt(s0, 1, g);
+                    }
+            };
 #pragma unroll
             for (int m = 0; m < NX; ++m) {
                 const int ks = m / (TT + 1);
@@ -701,25 +716,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     if (st < NRS) resp_step(e, st);
                     else dma(b + 2, s2, st - NRS);
                 }
+                if (m == NX - 3) ld_pair(0);
                 __builtin_amdgcn_sched_barrier(0);
             }
             h8 rhi[2], rlo[2];
             resp_ops(rhi, rlo);
-            // A operands, indexed g = 2 db + sh; the first pair's were read
-            // during phase 1, each pair's first MFMA reads the next pair's
-            h8 ru0[2 * DB], ru1[2 * DB];
-            auto ld_pair = [&](int i0) {           // operands of the pair starting at MFMA i0
-                const RtOp o = rt_op(i0);
-                const int npd = (o.db + 1 < DB && rt_op(i0 + 1).db != o.db) ? 2 : 1;
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-                    if (x < npd) {
-                        const int g = 2 * (o.db + x) + o.sh;
-                        ru0[g] = ldTt(s0, 0, g);
-                        if (TT == 2) ru1[g] = ldTt(s0, 1, g);
-                    }
-            };
-            ld_pair(0);
             f4v pb[4];
             float en[16];
 #pragma unroll
