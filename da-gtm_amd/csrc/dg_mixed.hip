@@ -190,11 +190,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     float t1 = 0.f, t2 = 0.f, nmit = 0.f;
     // exponents of a tile from its accumulator, the tile maximum, the
     // underflow mask when it can matter, and the online rescaling
-    auto tile_exp = [&](int s, const f16v &c, float (&z)[16]) {
+    auto tile_exp = [&](const f4v (&akr)[4], const f16v &c, float (&z)[16]) {
         float zt = -INFINITY;
 #pragma unroll
         for (int m4 = 0; m4 < 4; ++m4) {
-            const f4v ak = ldK(s, m4);
+            const f4v ak = akr[m4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 z[4 * m4 + q] = fmaf(c1, c[4 * m4 + q], ak[q]);
@@ -264,7 +264,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int ks = mm / (TT + 1);
             xmfma(mm, ldY(0, 0, ks), ldY(0, 1, ks), c);
         }
-        tile_exp(0, c, z);
+        f4v ak0[4];
+#pragma unroll
+        for (int m4 = 0; m4 < 4; ++m4) ak0[m4] = ldK(0, m4);
+        tile_exp(ak0, c, z);
     }
     // iteration 0 refills tile 0's slot: every wave must be done reading it
     __syncthreads();
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         f16v c;
 #pragma unroll
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
+        f4v akr[4];                                // tile t+1's aK
         h8 ya0[KS], ya1[KS];
 #pragma unroll
         for (int ks = 0; ks < 2 && ks < KS; ++ks) { ya0[ks] = ldY(s1, 0, ks); ya1[ks] = ldY(s1, 1, ks); }
@@ -288,6 +292,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
                 ya1[ks + 2] = ldY(s1, 1, ks + 2);
             }
             xmfma(mm, ya0[ks], ya1[ks], c);
+            // tile t+1's aK goes to registers in the slot before the first DMA:
+            // after an LDS-DMA the compiler puts vmcnt(0) (the DMA's completion)
+            // before any LDS read it cannot prove disjoint from the DMA target
+            if (mm == NRW * NX / W1 - 1) {
+#pragma unroll
+                for (int m4 = 0; m4 < 4; ++m4) akr[m4] = ldK(s1, m4);
+            }
 #pragma unroll
             for (int st = mm * W1 / NX; st < (mm + 1) * W1 / NX; ++st) {
                 if (st < NRW) row_sum(z, st);
@@ -296,7 +307,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
             __builtin_amdgcn_sched_barrier(0);
         }
         tile_close();
-        tile_exp(s1, c, z);
+        tile_exp(akr, c, z);
         __syncthreads();                           // also waits for this iteration's DMA
     }
 #pragma unroll

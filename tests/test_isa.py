@@ -135,3 +135,23 @@ def test_pass2_loop_has_no_vector_scalar_loads(asm):
     for name, body in _kernels(asm, "k_e2_mfma").items():
         n = sum(1 for x in body if x.startswith("global_load_dword "))
         assert n <= 1, (name, n)
+
+
+def test_pass1_loop_waits_for_dma_only_at_the_barrier(asm):
+    """Pass 1 reads everything it needs from LDS before it issues the tile's
+    LDS-DMA, so every vmcnt wait in each instantiation belongs to a barrier
+    (prologue and loop end): no LDS read sits between such a wait and the
+    next s_barrier."""
+    for name, body in _kernels(asm, "k_e1_mfma").items():
+        for j, x in enumerate(body):
+            if x.startswith("s_waitcnt") and "vmcnt" in x:
+                nxt = next((y for y in body[j + 1:] if y.startswith(("s_barrier", "ds_read"))), None)
+                assert nxt is not None and nxt.startswith("s_barrier"), (name, body[j - 2:j + 3])
+
+
+def test_dma_wait_checker_sees_an_early_wait():
+    """The checker itself: a vmcnt wait followed by an LDS read before the
+    barrier is what it must reject."""
+    body = ["global_load_lds_dwordx4 v1, s[2:3]", "s_waitcnt vmcnt(0)", "ds_read_b128 v[4:7], v2", "s_barrier"]
+    nxt = next((y for y in body[2:] if y.startswith(("s_barrier", "ds_read"))), None)
+    assert nxt.startswith("ds_read")
