@@ -111,7 +111,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int NS = 2;
     constexpr int NX = (TT + 1) * KS;              // cross-term MFMAs per tile
     static_assert(NS * SLOT * 2 <= 80 * 1024, "two workgroups per CU");
-    __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
+    // one object per ring slot: with the slot a constant in each unrolled
+    // iteration, the compiler sees DMA targets and operand reads as distinct
+    __shared__ __attribute__((aligned(1024))) _Float16 smem0[SLOT];
+    __shared__ __attribute__((aligned(1024))) _Float16 smem1[SLOT];
+    static_assert(NS == 2, "two slot objects");
+    auto sp = [&](int s) -> _Float16 * { return s ? smem1 : smem0; };
     __shared__ double wsum[3][4];
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
@@ -159,7 +164,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int I = wu + 4 * i;
         __builtin_amdgcn_global_load_lds(
             (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
-            (__attribute__((address_space(3))) void *)&smem[s][I * 512], 16, 0, 0);
+            (__attribute__((address_space(3))) void *)&sp(s)[I * 512], 16, 0, 0);
     };
     // Marks the lane offsets as redefined once per loop iteration (an empty
     // asm, no instruction).  Otherwise the compiler hoists base + offset into
@@ -172,10 +177,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int i = 0; i < NIW; ++i) asm volatile("" : "+v"(voff[i]));
     };
     auto ldY = [&](int s, int t, int ks) {         // A operand: centre row j, term t
-        return *(const h8 *)&smem[s][(t * KT + j) * LR + 16 * ks + 8 * h];
+        return *(const h8 *)&sp(s)[(t * KT + j) * LR + 16 * ks + 8 * h];
     };
     auto ldK = [&](int s, int m4) {                // aK of rows 8 m4 + 4 h .. + 3
-        return *(const f4v *)&smem[s][OFF_AK + 2 * (8 * m4 + 4 * h)];
+        return *(const f4v *)&sp(s)[OFF_AK + 2 * (8 * m4 + 4 * h)];
     };
     // cross-term MFMA number m (m = (TT + 1) ks + which) of the tile whose
     // Y rows are a0 (hi) / a1 (lo)
@@ -274,9 +279,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     // work steps spread evenly over the MFMA slots: the row sums of tile t,
     // then the DMA loads (slot m runs steps m W1 / NX .. (m + 1) W1 / NX - 1)
     constexpr int W1 = NRW + NIW;
-    for (int t = 0; t + 1 < ntiles; ++t) {
+    // iteration t; the slots are template constants, the loop below runs two
+    // iterations per trip, one per slot rotation
+    auto iter1 = [&](const int t, auto S1c) {
+        constexpr int s1 = decltype(S1c)::value, s2 = (s1 + 1) % NS;
         dma_offsets_opaque();
-        const int s1 = (t + 1) % NS, s2 = (t + 2) % NS;
         f16v c;
 #pragma unroll
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
@@ -309,6 +316,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         tile_close();
         tile_exp(akr, c, z);
         __syncthreads();                           // also waits for this iteration's DMA
+    };
+    for (int t = 0; t + 1 < ntiles; t += 2) {
+        iter1(t, std::integral_constant<int, 1>{});
+        if (t + 2 >= ntiles) break;
+        iter1(t + 1, std::integral_constant<int, 0>{});
     }
 #pragma unroll
     for (int st = 0; st < NRW; ++st) row_sum(z, st);
