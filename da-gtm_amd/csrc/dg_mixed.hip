@@ -713,7 +713,9 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             }
             // R T A operands (block b's transposed rows, resident all iteration),
             // indexed g = 2 db + sh: the first pair's are read three MFMAs
-            // before phase 1 ends, each pair's first MFMA reads the next pair's
+            // before phase 1 ends (at its first MFMA when phase 1 has fewer
+            // than three), each pair's first MFMA reads the next pair's
+            constexpr int MLD = NX >= 3 ? NX - 3 : 0;
             h8 ru0[2 * DB], ru1[2 * DB];
             auto ld_pair = [&](int i0) {           // operands of the pair starting at MFMA i0
                 const RtOp o = rt_op(i0);
@@ -739,7 +741,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     if (st < NRS) resp_step(e, st);
                     else dma(b + 2, s2, st - NRS);
                 }
-                if (m == NX - 3) ld_pair(0);
+                if (m == MLD) ld_pair(0);
                 __builtin_amdgcn_sched_barrier(0);
             }
             h8 rhi[2], rlo[2];

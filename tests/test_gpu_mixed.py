@@ -70,6 +70,8 @@ def _fp_problem(seed, N, D, K, M, L):
     (5000, 166, 1000, 64, 3, 1.0),       # temperature 1 (one exponential)
     (3001, 64, 343, 27, 3, 2.0),         # ragged N and K, D = 64
     (777, 32, 100, 25, 2, 5.0),          # D = 32
+    (200000, 12, 400, 25, 2, 2.0),       # D <= 16 exact in FP16: two cross-term MFMAs per block, 14 blocks per chunk
+    (9000, 16, 216, 27, 3, 1.0),         # the same at temperature 1
 ])
 def test_binary_estep_mstep_vs_fp64(N, D, K, M, L, temp):
     _say(f"case: binary mixed vs fp64 N={N} D={D} K={K} temp={temp}")

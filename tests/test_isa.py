@@ -155,3 +155,26 @@ def test_dma_wait_checker_sees_an_early_wait():
     body = ["global_load_lds_dwordx4 v1, s[2:3]", "s_waitcnt vmcnt(0)", "ds_read_b128 v[4:7], v2", "s_barrier"]
     nxt = next((y for y in body[2:] if y.startswith(("s_barrier", "ds_read"))), None)
     assert nxt.startswith("ds_read")
+
+
+def _e2_lds_reads_expected(KS, TT):
+    """16-byte LDS reads of one pass-2 instantiation: the prologue block
+    (KS TT data-row operands, 4 pB and 4 mask-threshold vectors), three
+    unrolled loop iterations (the same plus the 2 DB TT transposed-row
+    operands of the R T product) and the epilogue block's R T operands."""
+    DB = (16 * KS + 31) // 32
+    return (KS * TT + 8) + 3 * (KS * TT + 2 * DB * TT + 8) + 2 * DB * TT
+
+
+def test_pass2_reads_every_operand(asm):
+    """Every pass-2 instantiation reads all of its R T A operands in each
+    unrolled loop iteration: a schedule slot that never comes (for instance
+    an operand load placed before the first cross-term MFMA of a phase with
+    fewer MFMAs than the offset) leaves an operand register unread and the
+    R T product reads stale data."""
+    for name, body in _kernels(asm, "k_e2_mfma").items():
+        m = re.search(r"k_e2_mfmaILi(\d+)ELi(\d+)ELb([01])ELb([01])E", name)
+        assert m, name
+        KS, TT = int(m.group(1)), int(m.group(2))
+        n = sum(1 for x in body if x.startswith("ds_read_b128"))
+        assert n == _e2_lds_reads_expected(KS, TT), (name, n, _e2_lds_reads_expected(KS, TT))
