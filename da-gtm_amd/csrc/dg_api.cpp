@@ -382,6 +382,31 @@ int dg_mstep(dg_ctx *c, double *beta_out) {
     return DG_OK;
 }
 
+namespace {
+// dg_anneal's steps for dg_anneal_ctl: the context's own E-step, M-step and
+// critical temperatures, and the progress callback with the current beta.
+struct AnnealSteps {
+    dg_ctx *c;
+    dg_iter_cb cb;
+    void *user;
+};
+int st_estep(void *u, int isDA, double temp, double *llh, double *nfree) {
+    dg_estep_out eo;
+    const int rc = dg_estep(static_cast<AnnealSteps *>(u)->c, isDA, temp, &eo);
+    *llh = eo.llh; *nfree = eo.nfree;
+    return rc;
+}
+int st_mstep(void *u) { return dg_mstep(static_cast<AnnealSteps *>(u)->c, nullptr); }
+int st_ctemp(void *u, double temp, double eps, double *ct) {
+    return dg_next_ctemp(static_cast<AnnealSteps *>(u)->c, temp, eps, nullptr, ct, nullptr);
+}
+int st_iter(void *u, int nloop, int lloop, double temp, double nfree, double llh, double diff) {
+    const AnnealSteps *a = static_cast<AnnealSteps *>(u);
+    if (a->cb) a->cb(a->user, nloop, lloop, temp, nfree, llh, diff, a->c->beta);
+    return DG_OK;
+}
+}  // namespace
+
 int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_iter_cb cb,
               void *user) {
     if (!c || !p || !res) return DG_EINVAL;
@@ -389,59 +414,11 @@ int dg_anneal(dg_ctx *c, const dg_anneal_params *p, dg_anneal_result *res, dg_it
     if (p->schedule != DG_SCHED_EXP && p->schedule != DG_SCHED_LINEAR &&
         p->schedule != DG_SCHED_AUTO)
         return dg_fail(c, DG_EINVAL, "dg_anneal: unknown schedule");
-    const int QN = 50;   // INIT_QUEUE_SIZE, dagtm.cxx:45
-    double Q[QN];
-    int qnext = 0;
-    double temp = p->isDA ? p->start_temp : 1.0;
-    int isLast = 0, nloop = 0, ntemps = 0, exit_maxloop = 0;
-    double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff = 0;
-    int rc;
-    auto qmean = [&]() { double s = 0; for (int i = 0; i < QN; ++i) s += Q[i]; return s / QN; };
-    auto absv = [](double a) { return (a > 0) ? a : -a; };
-    do {
-        if (temp < 1.0) { temp = 1.0; isLast = 1; }
-        for (int i = 0; i < QN; ++i) Q[i] = DBL_MAX;
-        qnext = 0;
-        ntemps++;
-        int lloop = 0;
-        while (true) {
-            dg_estep_out eo;
-            if ((rc = dg_estep(c, p->isDA, temp, &eo))) return rc;
-            llh = eo.llh; nfree = eo.nfree;
-            qual_old = qual;
-            lloop++; nloop++;
-            qual = p->isDA ? nfree : llh;
-            diff = qual - qual_old;
-            if (cb) cb(user, nloop, lloop, temp, nfree, llh, diff, c->beta);
-            if (absv(diff) < p->eps) break;
-            if (nloop > QN) {
-                const double old = qmean();
-                Q[qnext] = qual; qnext = (qnext + 1) % QN;
-                const double da = qmean() - old;
-                if (absv(da) < p->eps) break;
-            }
-            if ((rc = dg_mstep(c, nullptr))) return rc;
-            if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
-        }
-        if (p->isDA && !isLast) {
-            if (p->schedule == DG_SCHED_LINEAR) {
-                temp -= p->delta;
-            } else if (p->schedule == DG_SCHED_AUTO) {
-                // dagtm.cxx:1919-1935: the largest critical temperature
-                // below the current one, else one EXP step
-                double ct = 0.0;
-                if ((rc = dg_next_ctemp(c, temp, p->eps, nullptr, &ct, nullptr))) return rc;
-                temp = (ct >= temp) ? temp * p->alpha : ct;
-            } else {
-                temp *= p->alpha;
-            }
-        } else {
-            break;
-        }
-    } while (!isLast);
-    res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
-    res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual; res->beta = c->beta;
-    return DG_OK;
+    AnnealSteps steps{c, cb, user};
+    const dg_anneal_ops ops{st_estep, st_mstep, st_ctemp, st_iter, nullptr};
+    const int rc = dg_anneal_ctl(p, nullptr, &ops, &steps, res);
+    res->beta = c->beta;
+    return rc;
 }
 
 int dg_tune(dg_ctx *c, const char *table_path, int *nc_out) {

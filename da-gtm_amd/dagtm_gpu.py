@@ -14,14 +14,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdagtm_gpu.so")
 
-DG_OK, DG_EINVAL, DG_ENOMEM, DG_EHIP, DG_ESTATE, DG_ESOLVE = 0, 1, 2, 3, 4, 5
+DG_OK, DG_EINVAL, DG_ENOMEM, DG_EHIP, DG_ESTATE, DG_ESOLVE, DG_ESTOP = 0, 1, 2, 3, 4, 5, 6
 FP64, MIXED, MIXED_R1 = 0, 1, 2
 SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
-           "dg_get_timing", "dg_batch_anneal", "dg_tune"]
+           "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl"]
 
 
 class DgConfig(C.Structure):
@@ -35,7 +35,7 @@ class DgEstepOut(C.Structure):
 class DgAnnealParams(C.Structure):
     _fields_ = [("isDA", C.c_int), ("schedule", C.c_int), ("maxloop", C.c_int),
                 ("alpha", C.c_double), ("delta", C.c_double), ("eps", C.c_double),
-                ("start_temp", C.c_double)]
+                ("start_temp", C.c_double), ("nloop0", C.c_int)]
 
 
 class DgAnnealResult(C.Structure):
@@ -54,6 +54,26 @@ class DgBatchItem(C.Structure):
 
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                       C.c_double, C.c_double, C.c_double)
+
+# step callbacks of dg_anneal_ctl (dg_anneal_ops)
+ESTEP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double),
+                       C.POINTER(C.c_double))
+MSTEP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+CTEMP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double))
+ONITER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                        C.c_double, C.c_double)
+EXPAND_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int))
+
+
+class DgAnnealOps(C.Structure):
+    _fields_ = [("estep", ESTEP_FN), ("mstep", MSTEP_FN), ("next_ctemp", CTEMP_FN),
+                ("on_iter", ONITER_FN), ("expand", EXPAND_FN)]
+
+
+class DgHdaState(C.Structure):
+    _fields_ = [("enable", C.c_int), ("K", C.c_int), ("Kmax", C.c_int), ("eps2", C.c_double),
+                ("nexpanded", C.c_int)]
+
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _lib = None
@@ -98,8 +118,64 @@ def load():
     L.dg_batch_anneal.argtypes = [C.POINTER(DgConfig), C.c_int, C.POINTER(DgBatchItem), C.c_int,
                                   C.POINTER(DgAnnealResult), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]
+    L.dg_anneal_ctl.argtypes = [C.POINTER(DgAnnealParams), C.POINTER(DgHdaState),
+                                C.POINTER(DgAnnealOps), vp, C.POINTER(DgAnnealResult)]
     _lib = L
     return L
+
+
+def anneal_ctl(steps, isDA=True, schedule=SCHED_EXP, maxloop=0, alpha=0.99, delta=0.1,
+               eps=1e-6, start_temp=1.0, nloop0=0, hda=None):
+    """dg_anneal_ctl over a Python step object with estep(isDA, temp) ->
+    (llh, nfree), mstep(), and optionally next_ctemp(temp, eps) -> float,
+    on_iter(nloop, lloop, temp, nfree, llh, diff) -> int and expand(nloop,
+    temp, K) -> new K.  hda: None or dict(K=, Kmax=, eps2=).  A Python
+    exception in a step stops the loop (status 99) and is re-raised here.
+    Returns (status, DgAnnealResult, nexpanded, K)."""
+    L = load()
+    err = []
+
+    def guard(f):
+        def g(*a):
+            try:
+                return f(*a)
+            except BaseException as e:       # noqa: BLE001 - re-raised below
+                err.append(e)
+                return 99
+        return g
+
+    def _e(_u, isDA_, temp, pl, pf):
+        pl[0], pf[0] = steps.estep(isDA_, temp)
+        return 0
+
+    def _m(_u):
+        steps.mstep()
+        return 0
+
+    def _c(_u, temp, e, pc):
+        pc[0] = steps.next_ctemp(temp, e)
+        return 0
+
+    def _i(_u, nloop, lloop, temp, nfree, llh, diff):
+        return int(steps.on_iter(nloop, lloop, temp, nfree, llh, diff) or 0)
+
+    def _x(_u, nloop, temp, pK):
+        pK[0] = int(steps.expand(nloop, temp, pK[0]))
+        return 0
+
+    ops = DgAnnealOps(ESTEP_FN(guard(_e)), MSTEP_FN(guard(_m)),
+                      CTEMP_FN(guard(_c)) if hasattr(steps, "next_ctemp") else CTEMP_FN(),
+                      ONITER_FN(guard(_i)) if hasattr(steps, "on_iter") else ONITER_FN(),
+                      EXPAND_FN(guard(_x)) if hasattr(steps, "expand") else EXPAND_FN())
+    p = DgAnnealParams(isDA=int(isDA), schedule=schedule, maxloop=maxloop, alpha=alpha,
+                       delta=delta, eps=eps, start_temp=start_temp, nloop0=nloop0)
+    h = DgHdaState(enable=1, K=hda["K"], Kmax=hda["Kmax"], eps2=hda["eps2"]) if hda else None
+    r = DgAnnealResult()
+    rc = L.dg_anneal_ctl(C.byref(p), C.byref(h) if h is not None else None, C.byref(ops), None,
+                         C.byref(r))
+    if err:
+        raise err[0]
+    return rc, r, (h.nexpanded if h is not None else 0), (h.K if h is not None else 0)
 
 
 def _f64(a):

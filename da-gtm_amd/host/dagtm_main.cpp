@@ -13,8 +13,10 @@
 #include <csignal>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <ctime>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -330,118 +332,118 @@ int main(int argc, char **argv) {
         return ok ? 0 : 1;
     }
 
-    // ---- annealing loop (dagtm.cxx:1508-2206, AUTO / EXP / LINEAR / EM, HDA)
-    const int QN = 50;
-    std::vector<double> Q(QN);
-    std::vector<double> hT, hL, hF;
-    double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff = 0;
-    bool isLast = false;
-    // HDA state (dagtm.cxx:1504-1506): an armed expansion fires at the end of
-    // the next temperature if the llh gained over it is at least eps2
-    bool ready = false, do_expand = false, w_in_engine = false;
-    int nexpanded = 0;
-    std::vector<double> ctemp_k;
-    const bool fresh = o.restart.empty();
-    auto absv = [](double a) { return (a > 0) ? a : -a; };
-    do {
-        if (temp < 1.0) { temp = 1.0; isLast = true; }
-        std::printf("\nT= %.10g\n", temp);
-        std::fill(Q.begin(), Q.end(), DBL_MAX);
-        int qnext = 0, lloop = 0;
-        const double qual_before_temp = llh;
-        std::printf(">>> %15s %15s %15s %15s %15s %15s\n", "GSEQ", "SEQ", "K", "TEMP", "NFREE", "LLH");
-        while (true) {
-            dg_estep_out eo;
-            CHECK_DG(dg_estep(ctx, o.isDA, temp, &eo));
-            qual_old = qual;
-            lloop++; nloop++;
-            llh = eo.llh; nfree = eo.nfree;
-            qual = o.isDA ? nfree : llh;
-            diff = qual - qual_old;
-            std::printf("::: %15d %15d %15d %15.7f %15.7f %15.7f %15.7f\n", nloop, lloop, K, temp,
-                        o.isDA ? nfree : -llh, llh, diff);
-            std::fflush(stdout);
-            if (o.log_progress) {
-                hT.push_back(temp); hL.push_back(llh);
-                if (o.isDA) hF.push_back(nfree);
-            }
-            if ((o.checkpointing && o.ckp_every > 0 && nloop % o.ckp_every == 0) || g_force_ckp) {
-                if (!write_ckp(qual)) std::fprintf(stderr, "checkpoint write failed\n");
-                if (g_force_ckp) {
-                    std::fprintf(stderr, "Force checkpointing done\n");
-                    dg_destroy(ctx);
-                    return 0;
-                }
-            }
-            if (absv(diff) < o.eps) break;
-            if (nloop > QN) {
-                double old = 0;
-                for (double q : Q) old += q;
-                old /= QN;
-                Q[qnext] = qual; qnext = (qnext + 1) % QN;
-                double avg = 0;
-                for (double q : Q) avg += q;
-                avg /= QN;
-                if (absv(avg - old) < o.eps) break;
-            }
-            CHECK_DG(dg_mstep(ctx, &beta));
-            w_in_engine = true;
-            if (o.maxloop && nloop >= o.maxloop) { isLast = true; break; }
+    // ---- annealing loop (dagtm.cxx:1508-2206, AUTO / EXP / LINEAR / EM, HDA):
+    // the library's control (dg_anneal_ctl) calls back into the steps below
+    struct Run {
+        dg_ctx *ctx;
+        Opts *o;
+        long N;
+        int D, K;
+        int nloop;
+        double temp = 0.0, beta = 0.0, qual = 0.0;
+        bool w_in_engine = false, fresh = true;
+        Mat *X, *FI, *W;
+        std::vector<double> hT, hL, hF, ctemp_k;
+        std::function<bool(double)> write_ckp;
+    };
+    Run run{ctx, &o, N, D, K, nloop};
+    run.beta = beta;
+    run.fresh = o.restart.empty();
+    run.X = &X; run.FI = &FI; run.W = &W;
+    run.write_ckp = [&](double q) { nloop = run.nloop; temp = run.temp; return write_ckp(q); };
+    dg_anneal_ops ops{};
+    ops.estep = [](void *u, int isDA, double t, double *llh, double *nfree) {
+        Run *r = static_cast<Run *>(u);
+        dg_estep_out eo;
+        const int rc = dg_estep(r->ctx, isDA, t, &eo);
+        *llh = eo.llh; *nfree = eo.nfree;
+        return rc;
+    };
+    ops.mstep = [](void *u) {
+        Run *r = static_cast<Run *>(u);
+        const int rc = dg_mstep(r->ctx, &r->beta);
+        if (rc == DG_OK) r->w_in_engine = true;
+        return rc;
+    };
+    ops.next_ctemp = [](void *u, double t, double eps, double *ct) {
+        Run *r = static_cast<Run *>(u);
+        r->ctemp_k.assign(r->K, 0.0);
+        return dg_next_ctemp(r->ctx, t, eps, nullptr, ct, r->ctemp_k.data());
+    };
+    ops.on_iter = [](void *u, int nl, int lloop, double t, double nfree, double llh, double diff) {
+        Run *r = static_cast<Run *>(u);
+        const Opts &o = *r->o;
+        r->nloop = nl;
+        r->temp = t;
+        r->qual = o.isDA ? nfree : llh;
+        if (lloop == 1) {
+            std::printf("\nT= %.10g\n", t);
+            std::printf(">>> %15s %15s %15s %15s %15s %15s\n", "GSEQ", "SEQ", "K", "TEMP", "NFREE", "LLH");
         }
-        if (o.isDA && !isLast) {
-            if (o.schedule == DG_SCHED_LINEAR) {
-                temp -= o.delta;
-            } else if (o.schedule == DG_SCHED_AUTO) {
-                // dagtm.cxx:1919-1935
-                double ct = 0.0;
-                CHECK_DG(dg_next_ctemp(ctx, temp, o.eps, nullptr, &ct, nullptr));
-                temp = (ct >= temp) ? temp * o.alpha : ct;
-            } else {
-                if (o.hda && K < o.Kmax) {
-                    // dagtm.cxx:1912, 1939-1977
-                    const double qual_diff_at_temp = llh - qual_before_temp;
-                    if (ready) {
-                        if (qual_diff_at_temp < o.eps2) ready = false;
-                        else do_expand = true;
-                    }
-                    if (!ready) {
-                        double ct = 0.0;
-                        ctemp_k.assign(K, 0.0);
-                        CHECK_DG(dg_next_ctemp(ctx, temp, o.eps, nullptr, &ct, ctemp_k.data()));
-                        if (ct >= temp * o.alpha) ready = true;
-                    }
-                    if (do_expand) {
-                        do_expand = ready = false;
-                        ++nexpanded;
-                        if (!o.out.empty() && o.log_progress &&
-                            !write_expansion_snapshot(ctx, o, (long)hT.size() - 1, N, D, X, FI,
-                                                      w_in_engine ? nullptr : &W, ctemp_k, beta))
-                            std::fprintf(stderr, "HDA snapshot write failed\n");
-                        // dagtm.cxx:2101-2160: double the grid side, new grid and
-                        // basis, Y = FI W with the current W
-                        K = (int)std::lround(std::pow(std::lround(std::pow(K, 1.0 / o.L)) * 2.0, o.L));
-                        if (!dgh::setup_basis(o.L, K, o.M, o.s, X, FI)) {
-                            std::fprintf(stderr, "HDA: no grid for K=%d M=%d L=%d\n", K, o.M, o.L);
-                            dg_destroy(ctx);
-                            return 2;
-                        }
-                        if (!w_in_engine && !fresh) {
-                            std::fprintf(stderr, "HDA: expansion before any M-step after a restart\n");
-                            dg_destroy(ctx);
-                            return 2;
-                        }
-                        CHECK_DG(dg_set_basis(ctx, FI.data(), K, (int)FI.cols, FI.cols,
-                                              w_in_engine ? nullptr : W.data(), D));
-                        w_in_engine = true;
-                        std::printf("new K : %d\n", K);
-                    }
-                }
-                temp *= o.alpha;
-            }
-        } else {
-            break;
+        std::printf("::: %15d %15d %15d %15.7f %15.7f %15.7f %15.7f\n", r->nloop, lloop, r->K, t,
+                    o.isDA ? nfree : -llh, llh, diff);
+        std::fflush(stdout);
+        if (o.log_progress) {
+            r->hT.push_back(t); r->hL.push_back(llh);
+            if (o.isDA) r->hF.push_back(nfree);
         }
-    } while (!isLast);
+        if ((o.checkpointing && o.ckp_every > 0 && r->nloop % o.ckp_every == 0) || g_force_ckp) {
+            if (!r->write_ckp(r->qual)) std::fprintf(stderr, "checkpoint write failed\n");
+            if (g_force_ckp) {
+                std::fprintf(stderr, "Force checkpointing done\n");
+                return DG_ESTOP;
+            }
+        }
+        return DG_OK;
+    };
+    ops.expand = [](void *u, int, double, int *Kio) {
+        // dagtm.cxx:1983-2160: expansion snapshot, then double the grid side,
+        // new grid and basis, Y = FI W with the current W
+        Run *r = static_cast<Run *>(u);
+        const Opts &o = *r->o;
+        if (!o.out.empty() && o.log_progress &&
+            !write_expansion_snapshot(r->ctx, o, (long)r->hT.size() - 1, r->N, r->D, *r->X, *r->FI,
+                                      r->w_in_engine ? nullptr : r->W, r->ctemp_k, r->beta))
+            std::fprintf(stderr, "HDA snapshot write failed\n");
+        const int K = (int)std::lround(std::pow(std::lround(std::pow(*Kio, 1.0 / o.L)) * 2.0, o.L));
+        if (!dgh::setup_basis(o.L, K, o.M, o.s, *r->X, *r->FI)) {
+            std::fprintf(stderr, "HDA: no grid for K=%d M=%d L=%d\n", K, o.M, o.L);
+            return (int)DG_EINVAL;
+        }
+        if (!r->w_in_engine && !r->fresh) {
+            std::fprintf(stderr, "HDA: expansion before any M-step after a restart\n");
+            return (int)DG_ESTATE;
+        }
+        const int rc = dg_set_basis(r->ctx, r->FI->data(), K, (int)r->FI->cols, r->FI->cols,
+                                    r->w_in_engine ? nullptr : r->W->data(), r->D);
+        if (rc != DG_OK) return rc;
+        r->w_in_engine = true;
+        r->K = K;
+        *Kio = K;
+        std::printf("new K : %d\n", K);
+        return (int)DG_OK;
+    };
+    dg_anneal_params ap{o.isDA ? 1 : 0, o.schedule, o.maxloop, o.alpha, o.delta, o.eps, temp,
+                        nloop};
+    dg_hda_state hda{o.hda ? 1 : 0, K, o.Kmax, o.eps2, 0};
+    dg_anneal_result ar{};
+    rc = dg_anneal_ctl(&ap, &hda, &ops, &run, &ar);
+    if (rc == DG_ESTOP) {
+        dg_destroy(ctx);
+        return 0;
+    }
+    if (rc != DG_OK) {
+        std::fprintf(stderr, "annealing failed (%d): %s\n", rc, dg_last_error(ctx));
+        dg_destroy(ctx);
+        return 2;
+    }
+    nloop = ar.nloop;
+    temp = ar.temp;
+    beta = run.beta;
+    K = run.K;
+    const double llh = ar.llh, nfree = ar.nfree, qual = ar.qual;
+    const int nexpanded = hda.nexpanded;
+    std::vector<double> &hT = run.hT, &hL = run.hL, &hF = run.hF;
 
     std::printf("\nSummary: iter %d temp %.10g beta %.10g llh %.10g nfree %.10g\n", nloop, temp,
                 beta, llh, nfree);

@@ -107,6 +107,9 @@ typedef struct {
     double delta;       /* LINEAR step (dagtm.cxx:2169-2171) */
     double eps;         /* -e, convergence threshold on |diff| */
     double start_temp;  /* -S or the first critical temperature / alpha^10 */
+    int    nloop0;      /* iterations done before this run: 0, or the iter of the
+                           checkpoint a restart read (dagtm.cxx:1197-1212); nloop
+                           counts on from it and maxloop is absolute */
 } dg_anneal_params;
 
 typedef struct {
@@ -128,6 +131,66 @@ typedef void (*dg_iter_cb)(void *user, int nloop, int lloop, double temp,
  * below the current one (dagtm.cxx:1919-1935). */
 int dg_anneal(dg_ctx *ctx, const dg_anneal_params *p, dg_anneal_result *res,
               dg_iter_cb cb, void *user);
+
+/* ---- the annealing control over caller-supplied steps --------------------
+ * dg_anneal and the dagtm driver both run the reference's temperature loop
+ * (dagtm.cxx:1508-2206) through dg_anneal_ctl, which calls back for every
+ * step, so a driver with its own per-iteration work (progress rows,
+ * checkpoints, the forced checkpoint of dagtm.cxx:1609-1687) or with HDA
+ * grid doubling (dagtm.cxx:1938-2164) uses the same control.  Per
+ * temperature: clamp temp < 1 to 1 and mark the last pass (1510-1514),
+ * refill the 50-entry quality queue with DBL_MAX (1521), remember
+ * qual_before_temp = llh (1523); per iteration: E-step, qual_old = qual (0
+ * before the first), nloop++ and lloop++, qual = nfree (DA) or llh (EM),
+ * diff = qual - qual_old, on_iter, then break when |diff| < eps or, once the
+ * global nloop exceeds 50, when the queue mean moved by less than eps after
+ * pushing qual (1689-1709; both before the M-step); M-step; stop after it
+ * when nloop reaches maxloop (1790-1795).  After a temperature (DA, not
+ * the last): LINEAR temp -= delta; AUTO temp = next_ctemp's value, or temp *
+ * alpha when that is not below temp (1919-1935); EXP runs the HDA rule
+ * below, then temp *= alpha. */
+#define DG_ESTOP    6   /* a hook stopped dg_anneal_ctl (not a failure of the hook) */
+
+typedef struct {
+    /* required: one E-step at (isDA, temp) giving llh and nfree */
+    int (*estep)(void *user, int isDA, double temp, double *llh, double *nfree);
+    /* required: the M-step and beta update */
+    int (*mstep)(void *user);
+    /* AUTO schedule and HDA arming: the largest critical temperature below
+     * temp of the current state (dagtm_mpi_next_ctemp); may be NULL otherwise */
+    int (*next_ctemp)(void *user, double temp, double eps, double *ctemp_max);
+    /* optional, after each E-step's qualities and before the convergence
+     * tests; a nonzero return ends the run with that status (DG_ESTOP for a
+     * requested stop such as the forced checkpoint) */
+    int (*on_iter)(void *user, int nloop, int lloop, double temp, double nfree,
+                   double llh, double diff);
+    /* HDA: double the grid side, K <- (2 round(K^(1/L)))^L, and move the
+     * model to the new basis (dagtm.cxx:2101-2160); *K is in/out.  nloop is
+     * the iteration count at the expansion. */
+    int (*expand)(void *user, int nloop, double temp, int *K);
+} dg_anneal_ops;
+
+/* Hierarchical DA (-H, -k, -E; EXP schedule only, as in the reference).
+ * After a temperature that is not the last, while K < Kmax: an armed
+ * expansion fires when the llh gained over that temperature (llh -
+ * qual_before_temp, dagtm.cxx:1912) is at least eps2 and is cancelled
+ * otherwise; when not armed (or just cancelled), next_ctemp is called and
+ * the expansion is armed when its value is >= temp * alpha (1938-1977).  A
+ * firing calls expand. */
+typedef struct {
+    int    enable;      /* -H */
+    int    K;           /* in: current K; out: K after the expansions */
+    int    Kmax;        /* -k */
+    double eps2;        /* -E */
+    int    nexpanded;   /* out: expansions done */
+} dg_hda_state;
+
+/* Runs the loop with ops (hda may be NULL: no HDA).  Returns DG_OK, DG_EINVAL
+ * (missing hook, unknown schedule), or the first nonzero status a hook
+ * returned; res is filled in every case except DG_EINVAL, with res->beta = 0
+ * (beta belongs to the steps; dg_anneal fills in the context's). */
+int dg_anneal_ctl(const dg_anneal_params *p, dg_hda_state *hda, const dg_anneal_ops *ops,
+                  void *user, dg_anneal_result *res);
 
 /* Critical temperatures of the current state, for AUTO cooling.  For each
  * centre k, with S1 = sum_n R_kn x x^T and S2 = sum_n R_kn^2 x x^T over

@@ -653,32 +653,44 @@ static void q_push(qqueue *q, double x) { q->v[q->next] = x; q->next = (q->next 
 static double q_mean(const qqueue *q) { double s = 0; for (int i = 0; i < QUEUE_SIZE; i++) s += q->v[i]; return s / QUEUE_SIZE; }
 #define ABSV(a) (((a) > (0)) ? (a) : (-(a)))
 
-/* dagtm.cxx:1508-1795 + 1909-2206 for the AUTO, EXP and LINEAR schedules
- * and EM mode: clamp temp<1 to 1 and mark the last pass (1510-1514); reset
- * the quality queue per temperature (1521); per iteration resp, qual, the
- * |diff|<eps test and the 50-window moving-average test (1689-1709), both
- * breaking before the M-step; M-step + beta; maxloop (1790-1795); then the
- * next critical temperature (1919-1935), temp *= alpha or temp -= delta
- * (2166-2171).  qual_old starts at 0 (Q7). */
-int orc_anneal(orc_model *m, const orc_anneal_params *p,
-               orc_anneal_result *res, orc_iter_cb cb, void *user)
+/* dagtm.cxx:1508-1795 + 1909-2206, control only: the temperature loop
+ * (do { ... } while (!isLast)) with
+ *   - clamp temp < 1 to 1 and mark the last pass (1510-1514);
+ *   - the quality queue refilled with DBL_MAX per temperature (1521) and
+ *     qual_before_temp = llh (1523);
+ *   - per iteration (1531-1795): resp + qual (the E-step), qual_old = qual,
+ *     lloop++, nloop++, qual = nfree (DA) / llh (EM), diff = qual -
+ *     qual_old (qual starts at 0: Q7), the progress row and checkpoint
+ *     (on_iter), |diff| < eps -> break (1689-1693), nloop > 50 -> push qual
+ *     and break when |mean after - mean before| < eps (1696-1709), the
+ *     M-step, maxloop -> last pass (1790-1795);
+ *   - after the temperature, if DA and not the last: AUTO (1918-1934), EXP
+ *     with the HDA arm/cancel/fire rule (1936-2166), LINEAR (2168-2170);
+ *     else leave the loop (2176-2179). */
+int orc_anneal_ctl(const orc_anneal_params *p, orc_hda *hda, const orc_steps *st,
+                   void *user, orc_anneal_result *res)
 {
     double temp = p->isDA ? p->start_temp : 1.0;
-    int isLast = 0, nloop = 0, ntemps = 0, exit_maxloop = 0;
+    int isLast = 0, nloop = p->nloop0, ntemps = 0, exit_maxloop = 0, rc = 0;
+    int bReadyToExpand = 0, bDoExpand = 0;
     double qual = 0, qual_old = 0, llh = 0, nfree = 0, diff;
+    const int hda_on = hda && hda->enable;
     qqueue Q;
+    if (hda) hda->nexpanded = 0;
     do {
         if (temp < 1.0) { temp = 1.0; isLast = 1; }
         q_reset(&Q, DBL_MAX);
         ntemps++;
+        double qual_before_temp = llh;
         int lloop = 0;
         while (1) {
-            orc_estep(m, p->isDA, temp, &llh, &nfree);
+            if ((rc = st->estep(user, p->isDA, temp, &llh, &nfree))) goto out;
             qual_old = qual;
             lloop++; nloop++;
             qual = p->isDA ? nfree : llh;
             diff = qual - qual_old;
-            if (cb) cb(user, nloop, lloop, temp, nfree, llh, diff, m->beta);
+            if (st->on_iter && (rc = st->on_iter(user, nloop, lloop, temp, nfree, llh, diff)))
+                goto out;
             if (ABSV(diff) < p->eps) break;
             if (nloop > QUEUE_SIZE) {
                 double old = q_mean(&Q);
@@ -687,31 +699,82 @@ int orc_anneal(orc_model *m, const orc_anneal_params *p,
                 double da = avg - old;
                 if (ABSV(da) < p->eps) break;
             }
-            orc_mstep(m);
+            if ((rc = st->mstep(user))) goto out;
             if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
         }
+        double qual_diff_at_temp = llh - qual_before_temp;
         if (p->isDA && !isLast) {
-            if (p->schedule == ORC_SCHED_LINEAR) {
-                temp -= p->delta;
-            } else if (p->schedule == ORC_SCHED_AUTO) {
-                /* dagtm.cxx:1918-1934; R is the last E-step's, G the last
-                 * M-step's (the reference's vggsub is refreshed only after the
-                 * convergence test, dagtm.cxx:1715) */
-                double ct = orc_next_ctemp(m->T, m->N, m->D, m->Y, m->K, m->R, m->G, temp,
-                                           m->beta, p->eps, NULL);
+            if (p->schedule == ORC_SCHED_AUTO) {
+                double ct;
+                if ((rc = st->next_ctemp(user, temp, p->eps, &ct))) goto out;
                 if (ct >= temp) ct = temp * p->alpha;
                 temp = ct;
-            } else {
+            } else if (p->schedule == ORC_SCHED_EXP) {
+                if (hda_on && hda->K < hda->Kmax) {
+                    if (bReadyToExpand) {
+                        if (qual_diff_at_temp < hda->eps2) bReadyToExpand = 0;
+                        else bDoExpand = 1;
+                    }
+                    if (!bReadyToExpand) {
+                        double ct;
+                        if ((rc = st->next_ctemp(user, temp, p->eps, &ct))) goto out;
+                        if (ct >= temp * p->alpha) bReadyToExpand = 1;
+                    }
+                    if (bDoExpand) {
+                        bDoExpand = 0;
+                        bReadyToExpand = 0;
+                        hda->nexpanded++;
+                        if ((rc = st->expand(user, nloop, temp, &hda->K))) goto out;
+                    }
+                }
                 temp *= p->alpha;
+            } else if (p->schedule == ORC_SCHED_LINEAR) {
+                temp -= p->delta;
             }
         } else {
             break;
         }
     } while (!isLast);
+out:
     res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
     res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual;
-    res->beta = m->beta;
+    res->beta = 0;
+    return rc;
+}
+
+/* orc_anneal's steps: the materialised model */
+typedef struct { orc_model *m; orc_iter_cb cb; void *user; } orc_model_steps;
+static int ms_estep(void *u, int isDA, double temp, double *llh, double *nfree)
+{
+    orc_estep(((orc_model_steps *)u)->m, isDA, temp, llh, nfree);
     return 0;
+}
+static int ms_mstep(void *u) { orc_mstep(((orc_model_steps *)u)->m); return 0; }
+static int ms_ctemp(void *u, double temp, double eps, double *ct)
+{
+    /* R is the last E-step's, G the last M-step's (the reference's vggsub is
+     * refreshed only after the convergence test, dagtm.cxx:1715) */
+    orc_model *m = ((orc_model_steps *)u)->m;
+    *ct = orc_next_ctemp(m->T, m->N, m->D, m->Y, m->K, m->R, m->G, temp, m->beta, eps, NULL);
+    return 0;
+}
+static int ms_iter(void *u, int nloop, int lloop, double temp, double nfree, double llh,
+                   double diff)
+{
+    orc_model_steps *s = (orc_model_steps *)u;
+    if (s->cb) s->cb(s->user, nloop, lloop, temp, nfree, llh, diff, s->m->beta);
+    return 0;
+}
+
+/* The whole loop on the materialised model (no HDA). */
+int orc_anneal(orc_model *m, const orc_anneal_params *p,
+               orc_anneal_result *res, orc_iter_cb cb, void *user)
+{
+    orc_model_steps s = {m, cb, user};
+    orc_steps st = {ms_estep, ms_mstep, ms_ctemp, ms_iter, NULL};
+    int rc = orc_anneal_ctl(p, NULL, &st, &s, res);
+    res->beta = m->beta;
+    return rc;
 }
 
 double orc_next_ctemp(const double *T, long N, int D, const double *Y, int K,

@@ -110,6 +110,7 @@ int  orc_mstep(orc_model *m);
 typedef struct {
     int isDA, schedule, maxloop;
     double alpha, delta, eps, start_temp;
+    int nloop0;         /* restart: the checkpoint's iter (dagtm.cxx:1197-1212) */
 } orc_anneal_params;
 
 typedef struct {
@@ -128,6 +129,26 @@ typedef void (*orc_iter_cb)(void *user, int nloop, int lloop, double temp,
 
 int orc_anneal(orc_model *m, const orc_anneal_params *p,
                orc_anneal_result *res, orc_iter_cb cb, void *user);
+
+/* The loop control of dagtm.cxx:1508-2206 alone, over step callbacks, so
+ * that it can be driven by scripted qualities.  The steps are called in the
+ * reference's order; ev (may be NULL) receives every event. */
+enum { ORC_EV_ESTEP = 1, ORC_EV_MSTEP = 2, ORC_EV_CTEMP = 3, ORC_EV_EXPAND = 4 };
+typedef struct {
+    int  (*estep)(void *user, int isDA, double temp, double *llh, double *nfree);
+    int  (*mstep)(void *user);
+    int  (*next_ctemp)(void *user, double temp, double eps, double *ctemp_max);
+    int  (*on_iter)(void *user, int nloop, int lloop, double temp, double nfree,
+                    double llh, double diff);
+    int  (*expand)(void *user, int nloop, double temp, int *K);
+} orc_steps;
+
+/* HDA (dagtm.cxx:1938-2164, EXP schedule): enable, K (in/out), Kmax, eps2,
+ * nexpanded (out) */
+typedef struct { int enable, K, Kmax; double eps2; int nexpanded; } orc_hda;
+
+int orc_anneal_ctl(const orc_anneal_params *p, orc_hda *hda, const orc_steps *st,
+                   void *user, orc_anneal_result *res);
 
 /* Next critical temperature below temp (dagtm_mpi_next_ctemp and its helper,
  * mpicomm.c:561-675): per centre k, x_n = t_n - y_k, S1 = sum_n R_kn x x^T,

@@ -8,7 +8,6 @@ Every wrapper takes and returns C-contiguous float64 numpy arrays.
 """
 import ctypes as C
 import os
-import sys
 
 import numpy as np
 
@@ -33,7 +32,7 @@ class OrcModel(C.Structure):
 class OrcAnnealParams(C.Structure):
     _fields_ = [("isDA", C.c_int), ("schedule", C.c_int), ("maxloop", C.c_int),
                 ("alpha", C.c_double), ("delta", C.c_double), ("eps", C.c_double),
-                ("start_temp", C.c_double)]
+                ("start_temp", C.c_double), ("nloop0", C.c_int)]
 
 
 class OrcAnnealResult(C.Structure):
@@ -44,6 +43,25 @@ class OrcAnnealResult(C.Structure):
 
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                       C.c_double, C.c_double, C.c_double)
+
+# step callbacks of orc_anneal_ctl
+ESTEP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double),
+                       C.POINTER(C.c_double))
+MSTEP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+CTEMP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double))
+ONITER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                        C.c_double, C.c_double)
+EXPAND_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int))
+
+
+class OrcSteps(C.Structure):
+    _fields_ = [("estep", ESTEP_FN), ("mstep", MSTEP_FN), ("next_ctemp", CTEMP_FN),
+                ("on_iter", ONITER_FN), ("expand", EXPAND_FN)]
+
+
+class OrcHda(C.Structure):
+    _fields_ = [("enable", C.c_int), ("K", C.c_int), ("Kmax", C.c_int), ("eps2", C.c_double),
+                ("nexpanded", C.c_int)]
 
 SOLVE_PINV, SOLVE_CHOL = 0, 1
 SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
@@ -93,6 +111,9 @@ def lib():
         L.orc_mstep.argtypes = [C.POINTER(OrcModel)]
         L.orc_anneal.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcAnnealParams),
                                  C.POINTER(OrcAnnealResult), ITER_CB, C.c_void_p]
+        L.orc_anneal_ctl.argtypes = [C.POINTER(OrcAnnealParams), C.POINTER(OrcHda),
+                                     C.POINTER(OrcSteps), C.c_void_p,
+                                     C.POINTER(OrcAnnealResult)]
         L.orc_map.argtypes = [C.POINTER(OrcModel), _dp, C.c_int, _dp]
         L.orc_next_ctemp.argtypes = [_dp, C.c_long, C.c_int, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _dp]
@@ -312,9 +333,104 @@ class Model:
         return Mo
 
 
+def anneal_ctl(steps, isDA=True, schedule=SCHED_EXP, maxloop=0, alpha=0.99, delta=0.1,
+               eps=1e-6, start_temp=1.0, nloop0=0, hda=None):
+    """orc_anneal_ctl over a Python step object with estep(isDA, temp) ->
+    (llh, nfree), mstep(), and optionally next_ctemp(temp, eps) -> float,
+    on_iter(nloop, lloop, temp, nfree, llh, diff) -> int and expand(nloop,
+    temp, K) -> new K.  hda: None or dict(K=, Kmax=, eps2=).  A Python
+    exception in a step stops the loop (status 99) and is re-raised here.
+    Returns (status, OrcAnnealResult, nexpanded, K)."""
+    err = []
+
+    def guard(f):
+        def g(*a):
+            try:
+                return f(*a)
+            except BaseException as e:       # noqa: BLE001 - re-raised below
+                err.append(e)
+                return 99
+        return g
+
+    def _e(_u, isDA_, temp, pl, pf):
+        pl[0], pf[0] = steps.estep(isDA_, temp)
+        return 0
+
+    def _m(_u):
+        steps.mstep()
+        return 0
+
+    def _c(_u, temp, e, pc):
+        pc[0] = steps.next_ctemp(temp, e)
+        return 0
+
+    def _i(_u, nloop, lloop, temp, nfree, llh, diff):
+        return int(steps.on_iter(nloop, lloop, temp, nfree, llh, diff) or 0)
+
+    def _x(_u, nloop, temp, pK):
+        pK[0] = int(steps.expand(nloop, temp, pK[0]))
+        return 0
+
+    st = OrcSteps(ESTEP_FN(guard(_e)), MSTEP_FN(guard(_m)),
+                  CTEMP_FN(guard(_c)) if hasattr(steps, "next_ctemp") else CTEMP_FN(),
+                  ONITER_FN(guard(_i)) if hasattr(steps, "on_iter") else ONITER_FN(),
+                  EXPAND_FN(guard(_x)) if hasattr(steps, "expand") else EXPAND_FN())
+    p = OrcAnnealParams(isDA=int(isDA), schedule=schedule, maxloop=maxloop, alpha=alpha,
+                        delta=delta, eps=eps, start_temp=start_temp, nloop0=nloop0)
+    h = OrcHda(enable=1, K=hda["K"], Kmax=hda["Kmax"], eps2=hda["eps2"]) if hda else None
+    r = OrcAnnealResult()
+    rc = lib().orc_anneal_ctl(C.byref(p), C.byref(h) if h is not None else None, C.byref(st),
+                              None, C.byref(r))
+    if err:
+        raise err[0]
+    return rc, r, (h.nexpanded if h is not None else 0), (h.K if h is not None else 0)
+
+
+class _HdaSteps:
+    """Steps of anneal_hda: the materialised model, replaced at an expansion."""
+
+    def __init__(self, T, FI, Y, beta, W, L, M, s, lam, solver, X, on_expand):
+        self.T, self.FI, self.L, self.M, self.s = T, FI, L, M, s
+        self.lam, self.solver, self.X, self.on_expand = lam, solver, X, on_expand
+        self.m = Model(T, FI, Y, beta, lam=lam, solver=solver)
+        self.Wlast = _f64(W).copy()
+        self.K = FI.shape[0]
+        self.ctemp_k = None
+        self.rows = []
+
+    def estep(self, isDA, temp):
+        llh, nfree, _ = self.m.estep(isDA, temp)
+        self._beta_used = self.m.beta
+        return llh, nfree
+
+    def mstep(self):
+        self.m.mstep()
+        self.Wlast = self.m.W.copy()
+
+    def next_ctemp(self, temp, eps):
+        c, self.ctemp_k = self.m.next_ctemp(temp, eps)
+        return c
+
+    def on_iter(self, nloop, lloop, temp, nfree, llh, diff):
+        self.rows.append((nloop, lloop, self.K, temp, nfree, llh, diff, self._beta_used))
+        return 0
+
+    def expand(self, nloop, temp, K):
+        if self.on_expand is not None:
+            self.on_expand(nloop - 1, self.m.map(self.X), self.X, self.m.Y.copy(), self.FI,
+                           self.Wlast, self.ctemp_k, self.m.beta)
+        g = int(round(K ** (1.0 / self.L))) * 2
+        self.K = g ** self.L
+        self.X, self.FI = stp(self.L, self.K, self.M, self.s)
+        self.m = Model(self.T, self.FI, self.FI @ self.Wlast, self.m.beta, lam=self.lam,
+                       solver=self.solver)
+        return self.K
+
+
 def anneal_hda(T, FI, Y, beta, W, L, M, s, Kmax=32768, eps2=1e-3, alpha=0.99, eps=1e-6,
                start_temp=1.0, maxloop=0, lam=0.0, solver=SOLVE_PINV, X=None, on_expand=None):
-    """Hierarchical DA: the EXP loop of dagtm.cxx:1508-1795, 1909-2164 with -H.
+    """Hierarchical DA: the EXP loop of dagtm.cxx:1508-1795, 1909-2164 with -H,
+    run by orc_anneal_ctl over the materialised model.
 
     After each temperature (not the last) and while K < Kmax: if an expansion
     was armed, it fires when the llh gained over the temperature
@@ -328,71 +444,15 @@ def anneal_hda(T, FI, Y, beta, W, L, M, s, Kmax=32768, eps2=1e-3, alpha=0.99, ep
     idx = nloop - 1.  Returns (summary dict, rows), each row (nloop, lloop, K,
     temp, nfree, llh, diff, beta)."""
     T = _f64(T)
-    N, D = T.shape
     K = FI.shape[0]
     if X is None:
         X, _ = stp(L, K, M, s)
-    m = Model(T, FI, Y, beta, lam=lam, solver=solver)
-    Wlast = _f64(W).copy()
-    temp = start_temp
-    isLast = False
-    nloop = ntemps = nexp = 0
-    qual = llh = nfree = 0.0
-    ready = do_expand = False
-    ctemp_k = None
-    rows = []
-    QN = 50
-    while True:
-        if temp < 1.0:
-            temp, isLast = 1.0, True
-        Q = [sys.float_info.max] * QN
-        qn = 0
-        ntemps += 1
-        qual_before = llh
-        lloop = 0
-        while True:
-            llh, nfree, _ = m.estep(1, temp)
-            qual_old = qual
-            lloop += 1
-            nloop += 1
-            qual = nfree
-            diff = qual - qual_old
-            rows.append((nloop, lloop, K, temp, nfree, llh, diff, m.beta))
-            if abs(diff) < eps:
-                break
-            if nloop > QN:
-                old = sum(Q) / QN
-                Q[qn] = qual
-                qn = (qn + 1) % QN
-                if abs(sum(Q) / QN - old) < eps:
-                    break
-            m.mstep()
-            Wlast = m.W.copy()
-            if maxloop and nloop >= maxloop:
-                isLast = True
-                break
-        if isLast:
-            break
-        qdiff = llh - qual_before
-        if K < Kmax:
-            if ready:
-                if qdiff < eps2:
-                    ready = False
-                else:
-                    do_expand = True
-            if not ready:
-                c, ctemp_k = m.next_ctemp(temp, eps)
-                if c >= temp * alpha:
-                    ready = True
-            if do_expand:
-                do_expand = ready = False
-                nexp += 1
-                if on_expand is not None:
-                    on_expand(nloop - 1, m.map(X), X, m.Y.copy(), FI, Wlast, ctemp_k, m.beta)
-                g = int(round(K ** (1.0 / L))) * 2
-                K = g ** L
-                X, FI = stp(L, K, M, s)
-                m = Model(T, FI, FI @ Wlast, m.beta, lam=lam, solver=solver)
-        temp *= alpha
-    return dict(nloop=nloop, ntemps=ntemps, nexp=nexp, K=K, temp=temp, llh=llh, nfree=nfree,
-                beta=m.beta, Y=m.Y.copy(), X=X, FI=FI, W=Wlast), rows
+    st = _HdaSteps(T, _f64(FI), Y, beta, W, L, M, s, lam, solver, X, on_expand)
+    rc, r, nexp, Kout = anneal_ctl(st, isDA=True, schedule=SCHED_EXP, maxloop=maxloop,
+                                   alpha=alpha, eps=eps, start_temp=start_temp,
+                                   hda=dict(K=K, Kmax=Kmax, eps2=eps2))
+    if rc:
+        raise RuntimeError("orc_anneal_ctl failed: %d" % rc)
+    return dict(nloop=r.nloop, ntemps=r.ntemps, nexp=nexp, K=Kout, temp=r.temp, llh=r.llh,
+                nfree=r.nfree, beta=st.m.beta, Y=st.m.Y.copy(), X=st.X, FI=st.FI,
+                W=st.Wlast), st.rows
