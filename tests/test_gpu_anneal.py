@@ -97,3 +97,53 @@ def test_map_and_resp_at_temperature(prec):
     assert np.abs(Mg - mo.map(X)).max() < 1e-12
     if prec == dg.FP64:                            # and the same state as the oracle's run
         assert np.abs(Y - m.Y).max() < 1e-8 * np.abs(m.Y).max()
+
+
+def test_window_break_matches_oracle():
+    """The run that ends on the 50-iteration moving-average window
+    (dagtm.cxx:1696-1709) rather than on |diff|: EM on the N=1000, D=12
+    mixture with K=400, M+1=26, s=1, lambda=1 and eps=1e-8 breaks at nloop 274
+    in the oracle (tests/test_oracle.py pins why); the FP64 path must break at
+    the same iteration, on the window, with the same qualities."""
+    _say("case: window break, EM N=1000 D=12 K=400 lambda=1 eps=1e-8")
+    T, X, FI, Y0, b0 = _problem(42, 1000, 12, 400, 25, 2)
+    kw = dict(isDA=False, eps=1e-8)
+    ro, rows_o = O.Model(T, FI, Y0, b0, lam=1.0, solver=O.SOLVE_CHOL).anneal(**kw)
+    with dg.Engine(precision=dg.FP64, lam=1.0) as e:
+        e.upload(T, FI, Y0, b0)
+        rg, rows_g = e.anneal(**kw)
+    _say(f"  nloop gpu {rg.nloop} oracle {ro.nloop}")
+    assert ro.nloop == 274 and not ro.exit_maxloop
+    _compare(rg, rows_g, ro, rows_o)
+    q = np.array([r[4] for r in rows_g])
+    d = np.abs(np.diff(np.concatenate([[0.0], q])))
+    assert d.min() >= 1e-8                        # never the |diff| test: the window broke
+
+
+@pytest.mark.parametrize("prec", [dg.FP64, dg.MIXED])
+def test_closed_form_iteration_counts(prec):
+    """eps = 1e300: one E-step and no M-step per temperature, EXP from 8 by
+    0.5 visits 8, 4, 2, 1, 1, and Y and beta come back bit-identical.  eps = 0
+    with maxloop 7: seven E+M iterations, the state equal bit for bit to seven
+    E-step/M-step calls."""
+    _say(f"case: closed-form iteration counts, precision {prec}")
+    T, X, FI, Y0, b0 = _problem(42, 1000, 12, 400, 25, 2)
+    with dg.Engine(precision=prec, lam=1e-3) as e:
+        e.upload(T, FI, Y0, b0)
+        r, rows = e.anneal(isDA=True, alpha=0.5, eps=1e300, start_temp=8.0)
+        assert [x[2] for x in rows] == [8.0, 4.0, 2.0, 1.0, 1.0]
+        assert r.nloop == 5 and r.ntemps == 5
+        _, Y, beta = e.download()
+        assert np.array_equal(Y, Y0) and beta == b0
+    with dg.Engine(precision=prec, lam=1e-3) as e:
+        e.upload(T, FI, Y0, b0)
+        r, rows = e.anneal(isDA=True, alpha=0.9, eps=0.0, start_temp=3.0, maxloop=7)
+        assert r.nloop == 7 and r.exit_maxloop == 1 and len(rows) == 7
+        W1, Y1, b1 = e.download()
+    with dg.Engine(precision=prec, lam=1e-3) as e:
+        e.upload(T, FI, Y0, b0)
+        for _ in range(7):
+            e.estep(True, 3.0)
+            e.mstep()
+        W2, Y2, b2 = e.download()
+    assert np.array_equal(Y1, Y2) and np.array_equal(W1, W2) and b1 == b2

@@ -261,3 +261,94 @@ def test_anneal_loop_semantics():
     m2 = O.Model(T, FI, Y, b0, solver=O.SOLVE_CHOL)
     res2, rows2 = m2.anneal(isDA=False, maxloop=30)
     assert all(r[2] == 1.0 for r in rows2) and res2.ntemps == 1
+
+
+def _c1_model(lam=1e-3, K=400, M=25):
+    import datagen
+    T = datagen.gmm(1000, 12, 42)
+    X, FI = datagen.latent_basis(2, K, M, 1.0)
+    W0, Y0, b0 = datagen.pca_init(T, FI, X)
+    return T, FI, Y0, b0, O.Model(T, FI, Y0, b0, lam=lam, solver=O.SOLVE_CHOL)
+
+
+def test_anneal_huge_eps_takes_no_mstep():
+    """eps = 1e300: every temperature ends on its first |diff| test, so each
+    runs one E-step and no M-step, Y and beta stay bit-identical, and EXP from
+    8 by 0.5 visits 8, 4, 2, 1, 1 (1 is not below 1; 0.5 is clamped to 1 and
+    is the last): five temperatures."""
+    T, FI, Y0, b0, m = _c1_model()
+    res, rows = m.anneal(isDA=True, alpha=0.5, eps=1e300, start_temp=8.0)
+    assert [r[2] for r in rows] == [8.0, 4.0, 2.0, 1.0, 1.0]
+    assert res.nloop == 5 and res.ntemps == 5 and not res.exit_maxloop
+    assert np.array_equal(m.Y, Y0) and m.beta == b0
+    # LINEAR from 2 by 0.4: 2, 1.6, 1.2, 0.8 -> 1 (four temperatures)
+    T, FI, Y0, b0, m = _c1_model()
+    res, rows = m.anneal(isDA=True, schedule=O.SCHED_LINEAR, delta=0.4, eps=1e300,
+                         start_temp=2.0)
+    assert [round(r[2], 12) for r in rows] == [2.0, 1.6, 1.2, 1.0]
+    assert np.array_equal(m.Y, Y0)
+
+
+def test_anneal_eps_zero_runs_maxloop_iterations():
+    """eps = 0 never breaks: exactly maxloop E-steps and maxloop M-steps (the
+    state equals maxloop manual E+M iterations, bit for bit), exit_maxloop."""
+    T, FI, Y0, b0, m = _c1_model()
+    res, rows = m.anneal(isDA=True, alpha=0.9, eps=0.0, start_temp=3.0, maxloop=7)
+    assert res.nloop == 7 and res.exit_maxloop == 1 and res.ntemps == 1 and len(rows) == 7
+    m2 = O.Model(T, FI, Y0, b0, lam=1e-3, solver=O.SOLVE_CHOL)
+    for _ in range(7):
+        m2.estep(1, 3.0)
+        m2.mstep()
+    assert np.array_equal(m.Y, m2.Y) and m.beta == m2.beta
+
+
+WINDOW_CASE = dict(lam=1.0, eps=1e-8, nloop=274)
+
+
+def _window_replay(q, eps, QN=50):
+    """The break rule applied to a recorded quality sequence (EM mode, one
+    temperature): returns (break index, 'diff' | 'window')."""
+    Q = [float(np.finfo(np.float64).max)] * QN
+    nxt, prev = 0, 0.0
+    with np.errstate(over="ignore", invalid="ignore"):
+        for i, x in enumerate(q):
+            n = i + 1
+            if abs(x - prev) < eps:
+                return n, "diff"
+            prev = x
+            if n > QN:
+                old = np.float64(0.0)
+                for v in Q:
+                    old += v
+                Q[nxt] = x
+                nxt = (nxt + 1) % QN
+                new = np.float64(0.0)
+                for v in Q:
+                    new += v
+                if abs(new / QN - old / QN) < eps:
+                    return n, "window"
+    return None, None
+
+
+def test_window_break_on_real_dynamics():
+    """A run that ends on the 50-iteration window, not on |diff|: EM on the
+    N=1000, D=12 mixture, K=400, M+1=26, s=1, lambda=1 (the ridge lets the
+    llh peak near iteration 243 and then fall slowly, so |diff| stays above
+    1.5e-8 while the 50-step mean change crosses zero).  With eps = 1e-8 the
+    window fires at nloop 274 (|mean change| 4.8e-9 there, 5.0e-8 one step
+    earlier); the GPU test runs the same case."""
+    T, FI, Y0, b0, m = _c1_model(lam=WINDOW_CASE["lam"])
+    res, rows = m.anneal(isDA=False, eps=WINDOW_CASE["eps"])
+    assert res.nloop == WINDOW_CASE["nloop"] and not res.exit_maxloop
+    q = [r[4] for r in rows]
+    assert _window_replay(q, WINDOW_CASE["eps"]) == (WINDOW_CASE["nloop"], "window")
+    d = np.abs(np.diff(np.concatenate([[0.0], q])))
+    assert d.min() > 1.5 * WINDOW_CASE["eps"]
+    assert int(np.argmax(q)) + 1 < WINDOW_CASE["nloop"]       # the llh peaked before
+
+
+def test_window_replay_checker():
+    """The replay rule itself on the scripted alternating sequence (breaks at
+    101 on the window) and on a sequence whose |diff| drops below eps."""
+    assert _window_replay([1.0 if n % 2 else -1.0 for n in range(1, 300)], 1e-3) == (101, "window")
+    assert _window_replay([1.0 - 0.5 ** n for n in range(1, 60)], 1e-3) == (10, "diff")
