@@ -131,6 +131,7 @@ int dg_mstep_run(dg_ctx *ctx);
 int dg_centres_from_w(dg_ctx *ctx);
 int dg_rownorms(dg_ctx *ctx, const double *X, long rows, int ld, int D, double *out);
 // dst (rows x ld, zero padded) from the dense rows x D src, both on the device.
-int dg_pad_rows(dg_ctx *ctx, const double *src, long rows, int D, int ld, double *dst);
+// dst[r][d] = src[r lds + d] for d < D, 0 for D <= d < ld (device arrays)
+int dg_pad_rows(dg_ctx *ctx, const double *src, long rows, int D, long lds, int ld, double *dst);
 int dg_reduce_logs(dg_ctx *ctx);
 int dg_sum_into(dg_ctx *ctx, const double *v, long n, int slot);

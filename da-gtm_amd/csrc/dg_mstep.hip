@@ -532,18 +532,19 @@ __global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ p
 
 // dst[r * ld + d] = src[r * D + d] for d < D, 0 for D <= d < ld.
 __global__ __launch_bounds__(256) void k_pad_rows(const double *__restrict__ src, long rows,
-                                                  int D, int ld, double *__restrict__ dst) {
+                                                  int D, long lds, int ld,
+                                                  double *__restrict__ dst) {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= rows * ld) return;
     const long r = e / ld;
     const int d = (int)(e % ld);
-    dst[e] = (d < D) ? src[r * D + d] : 0.0;
+    dst[e] = (d < D) ? src[r * lds + d] : 0.0;
 }
 
-int dg_pad_rows(dg_ctx *c, const double *src, long rows, int D, int ld, double *dst) {
+int dg_pad_rows(dg_ctx *c, const double *src, long rows, int D, long lds, int ld, double *dst) {
     const long n = rows * ld;
     hipLaunchKernelGGL(k_pad_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, src,
-                       rows, D, ld, dst);
+                       rows, D, lds, ld, dst);
     DG_HIP(c, hipGetLastError());
     return DG_OK;
 }
