@@ -205,6 +205,37 @@ int tune_find(const dg_ctx *c) {
     return it == g_tune.end() ? 0 : it->second;
 }
 
+// ---- dg_map_noalloc's scratch ---------------------------------------------
+// A file-scope device array: every GPU the library runs on holds its own copy
+// (16 MiB) until the process ends.  Its device address is looked up once per
+// device (hipGetSymbolAddress with that device current) and kept; host code
+// uses only that address, never the array's name.
+constexpr size_t kMapScratchDoubles = (size_t)1 << 21;
+__device__ __align__(256) double scratch[kMapScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// The calling thread has `device` current.  Returns this device's scratch
+// address, or nullptr when the lookup failed (reported by dg_map_noalloc).
+double *scratch_for_device(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) {
+            g_scratch_ptr[device] = static_cast<double *>(ptr);
+        } else {
+            (void)hipGetLastError();
+            g_scratch_ptr[device] = nullptr;
+        }
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -250,6 +281,7 @@ int dg_create(const dg_config *cfg, dg_ctx **out) {
         return DG_ENOMEM;
     }
     std::memset(c->h_scal, 0, kScalCount * sizeof(double));
+    c->map_scratch = scratch_for_device(c->device);
     *out = c;
     return DG_OK;
 }
@@ -721,6 +753,43 @@ int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
     dfree(dX); dfree(dM);
     if (e != hipSuccess) return dg_hip_check(c, e, "dg_map");
     return rc;
+}
+
+int dg_map_noalloc(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {
+    if (!c) return DG_EINVAL;
+    if (!c->loaded) return dg_fail(c, DG_ESTATE, "dg_map_noalloc before dg_upload");
+    if (!X || !M || L < 1 || L > 3 || ldX < L || ldM < L)
+        return dg_fail(c, DG_EINVAL, "dg_map_noalloc: bad args");
+    if (((size_t)c->K + (size_t)c->N) * (size_t)L > kMapScratchDoubles)
+        return dg_fail(c, DG_EINVAL,
+                       "dg_map_noalloc: (K + N) * L exceeds the scratch maximum of 2^21 "
+                       "doubles (use dg_map)");
+    if (!c->map_scratch)
+        return dg_fail(c, DG_EHIP, "dg_map_noalloc: no scratch address for this device");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    // the steps below launch kernels on buffers the context already owns
+    if (!c->have_estep) {
+        dg_estep_out eo;
+        if ((rc = dg_estep(c, c->last_isDA, c->last_temp, &eo))) return rc;
+    }
+    if (c->mixed() && (rc = dg_estep_fp64(c, c->last_isDA, c->last_temp))) return rc;
+    double *dX = c->map_scratch;                       // K x L
+    double *dM = c->map_scratch + (size_t)c->K * L;    // N x L
+    const size_t row = (size_t)L * sizeof(double);
+    if (ldX == L)
+        DG_HIP(c, hipMemcpyAsync(dX, X, (size_t)c->K * row, hipMemcpyHostToDevice, c->stream));
+    else
+        DG_HIP(c, hipMemcpy2DAsync(dX, row, X, (size_t)ldX * sizeof(double), row, (size_t)c->K,
+                                   hipMemcpyHostToDevice, c->stream));
+    if ((rc = dg_map_fp64(c, dX, L, dM))) return rc;
+    if (ldM == L)
+        DG_HIP(c, hipMemcpyAsync(M, dM, (size_t)c->N * row, hipMemcpyDeviceToHost, c->stream));
+    else
+        DG_HIP(c, hipMemcpy2DAsync(M, (size_t)ldM * sizeof(double), dM, row, row, (size_t)c->N,
+                                   hipMemcpyDeviceToHost, c->stream));
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
 }
 
 int dg_resp_dense(dg_ctx *c, double *R, long ldR) {

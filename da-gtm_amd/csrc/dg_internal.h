@@ -73,6 +73,7 @@ struct dg_ctx {
     float *partG32 = nullptr;      // NC x K
 
     int nc_override = 0;           // pass-2 chunk count chosen by the tuner (0: built-in)
+    double *map_scratch = nullptr; // this device's dg_map_noalloc scratch (static device array)
 
     // either mixed-precision E-step
     bool mixed() const { return cfg.precision != DG_FP64; }

@@ -21,7 +21,7 @@ SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
-           "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl"]
+           "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl", "dg_map_noalloc"]
 
 
 class DgConfig(C.Structure):
@@ -110,6 +110,7 @@ def load():
                             ITER_CB, vp]
     L.dg_next_ctemp.argtypes = [vp, C.c_double, C.c_double, vp, C.POINTER(C.c_double), vp]
     L.dg_map.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
+    L.dg_map_noalloc.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
     L.dg_resp_dense.argtypes = [vp, _dp, C.c_long]
     L.dg_download.argtypes = [vp, vp, C.c_long, vp, C.c_long, C.POINTER(C.c_double)]
     L.dg_get_stats.argtypes = [vp, vp, vp, C.c_long]
@@ -288,6 +289,13 @@ class Engine:
         X = _f64(X)
         Mo = np.zeros((self.N, X.shape[1]))
         self._chk(self.L.dg_map(self.h, X, X.shape[1], X.shape[1], Mo, X.shape[1]))
+        return Mo
+
+    def map_noalloc(self, X):
+        """dg_map through the library's fixed device scratch (no allocation)."""
+        X = _f64(X)
+        Mo = np.zeros((self.N, X.shape[1]))
+        self._chk(self.L.dg_map_noalloc(self.h, X, X.shape[1], X.shape[1], Mo, X.shape[1]))
         return Mo
 
     def resp_dense(self):

@@ -244,6 +244,18 @@ int dg_batch_anneal(const dg_config *cfg, int workers, const dg_batch_item *item
 int dg_map(dg_ctx *ctx, const double *X, int L, long ldX, double *M,
            long ldM);
 
+/* dg_map without device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, result and
+ * errors as dg_map; the latent coordinates (K x L) and the projection (N x L)
+ * are staged through a fixed device scratch array of 2^21 doubles (16 MiB,
+ * held by every GPU the library runs on until the process ends) instead of
+ * per-call buffers, so (K + N) * L must not exceed 2^21 (DG_EINVAL
+ * otherwise).  Every context on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (from different host threads) race. */
+int dg_map_noalloc(dg_ctx *ctx, const double *X, int L, long ldX, double *M,
+                   long ldM);
+
 /* Materialises R (K x N, ldR >= N) of the last E-step, in FP64 in both
  * precisions.  Debug / -R output (dagtm.cxx:2400-2414); intended for small
  * problems (K x N doubles are allocated on the device for the call). */
