@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """DA-GTM E+M iterations/s on one MI355X (mixed precision vs FP64 GPU).
 
-Workload (the headline configuration): N = 1,000,000 binary fingerprint
+Workload (the headline configuration, --config c3): N = 1,000,000 binary fingerprint
 vectors, D = 166 (50 prototypes, per-bit p ~ Beta(0.5, 2), flip 0.05,
 seed 7); L = 3 latent grid K = 20^3 = 8000; M = 5^3 = 125 RBF + bias
 (width = centre spacing, s = 1); ridge lambda = 1e-3.  One step = one E+M
@@ -38,8 +38,32 @@ import datagen  # noqa: E402
 
 METRIC = "DA-GTM E+M iter/s at N=1M,K=8k,D=166: mixed-prec vs FP64 GPU, same loglik"
 PEAK_FP16_TFLOPS = 2500.0      # MI355X dense FP16/BF16 MFMA
+PEAK_FP64_TFLOPS = 78.6        # MI355X FP64
+PEAK_HBM_GBPS = 8000.0
 TEMP = 2.0
 LAMBDA = 1e-3
+
+# single-instance configurations: (N, D, K, M, L, data, description)
+CONFIGS = {
+    "c1": (1000, 12, 400, 25, 2, "gmm",
+           "N=1000 Gaussian mixture D=12 (3 components, seed 42), L=2 K=400, M+1=26, s=1, "
+           "lambda=1e-3, DA temp 2.0; one step = one E+M iteration"),
+    "c2": (100_000, 166, 8000, 125, 3, "fp50",
+           "N=100,000 binary fingerprints D=166 (50 prototypes, flip 0.05, seed 7), L=3 K=8000, "
+           "M+1=126, s=1, lambda=1e-3, DA temp 2.0; one step = one E+M iteration"),
+    "c3": (1_000_000, 166, 8000, 125, 3, "fp50",
+           "N=1,000,000 binary fingerprints D=166 (50 prototypes, flip 0.05, seed 7), L=3 K=8000, "
+           "M+1=126, s=1, lambda=1e-3, DA temp 2.0; one step = one E+M iteration"),
+    "c5": (2_000_000, 166, 64000, 1000, 3, "fp200",
+           "N=2,000,000 binary fingerprints D=166 (200 prototypes, flip 0.05, seed 7), L=3 "
+           "K=64000, M+1=1001, s=1, lambda=1e-3, DA temp 2.0; one step = one E+M iteration"),
+}
+# CPU baseline samples per configuration: (points at all threads, points at
+# one thread); None = the whole workload, 0 = not measured.  A sample smaller
+# than the workload is timed at n and n / 2 points and the per-iteration time
+# a + b n extrapolated to the workload's N (the M-step's A = FI^T G FI and
+# solve do not grow with N; the rest does).
+CPU_SAMPLES = {"c1": (None, None), "c2": (None, 4000), "c3": (16000, 2000), "c5": (1000, 0)}
 
 
 def measured_traffic(kernel):
@@ -62,8 +86,12 @@ def measured_traffic(kernel):
         return None
 
 
-def build_problem(N, D, K, M, L, seed):
-    T = datagen.fingerprints(N, D, seed).astype(np.float64)
+def build_problem(N, D, K, M, L, data, seed=7):
+    if data == "gmm":
+        T = datagen.gmm(N, D, 42)
+    else:
+        T = datagen.fingerprints(N, D, seed, nproto=200 if data == "fp200" else 50)
+        T = T.astype(np.float64)
     X, FI = datagen.latent_basis(L, K, M, 1.0)
     W0, Y0, b0 = datagen.pca_init(T, FI, X)
     return T, X, FI, Y0, b0
@@ -84,24 +112,83 @@ def run_steps(eng, steps):
     return time.perf_counter() - t0, llh, phase
 
 
-def cpu_baseline(T, FI, Y0, b0, nsample, iters):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+def _oracle_time(O, T, FI, Y0, b0, n, threads, solver, iters=2):
+    """Seconds per E+M iteration of the oracle on the first n points (the
+    first iteration untimed when more than one is run)."""
     O.set_threads(threads)
-    Ts = np.ascontiguousarray(T[:nsample])
-    m = O.Model(Ts, FI, Y0, b0, lam=LAMBDA, solver=O.SOLVE_PINV)
+    m = O.Model(np.ascontiguousarray(T[:n]), FI, Y0, b0, lam=LAMBDA, solver=solver)
+    if iters > 1:
+        m.estep(1, TEMP)
+        m.mstep()
+        iters -= 1
     t0 = time.perf_counter()
     for _ in range(iters):
         m.estep(1, TEMP)
         m.mstep()
-    dt = (time.perf_counter() - t0) / iters
-    scale = T.shape[0] / nsample
-    return {"value": 1.0 / (dt * scale), "unit": "iter/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/liboracle.so (FP64, reference step order, materialised R) "
-                      f"on the first {nsample} of the {T.shape[0]} points, K={FI.shape[0]}, "
-                      f"D={T.shape[1]}, {iters} E+M iterations at {dt:.2f} s each, "
-                      f"scaled linearly in N to the full workload"}
+    return (time.perf_counter() - t0) / iters
+
+
+def _oracle_rate_at(O, T, FI, Y0, b0, n, threads, solver):
+    """(seconds per iteration extrapolated to the whole workload, note)."""
+    N = T.shape[0]
+    if n >= N:
+        t = _oracle_time(O, T, FI, Y0, b0, N, threads, solver)
+        return t, f"the whole workload, {t:.3f} s per iteration"
+    t_hi = _oracle_time(O, T, FI, Y0, b0, n, threads, solver)
+    t_lo = _oracle_time(O, T, FI, Y0, b0, n // 2, threads, solver)
+    b = max(t_hi - t_lo, 0.0) / (n - n // 2)
+    a = max(t_hi - b * n, 0.0)
+    return a + b * N, (f"the first {n} and {n // 2} points ({t_hi:.3f} and {t_lo:.3f} s per "
+                       f"iteration), a + b N fitted and extrapolated to N={N}")
+
+
+def cpu_baseline(cfg, T, FI, Y0, b0):
+    """The oracle (FP64, the reference's step order, materialised D and R) on
+    this box's host cores: all the threads this process may use (16 on a
+    one-GPU box) and one thread, each on a bounded sample of the workload.
+    Also compares one E+M iteration of both GPU paths with the oracle on the
+    all-thread sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    N = T.shape[0]
+    ns, ns1 = CPU_SAMPLES[cfg]
+    ns = N if ns is None else min(ns, N)
+    # the oracle's pseudo-inverse (Jacobi eigen-decomposition) of a 1001 x 1001
+    # A takes minutes; at M + 1 > 126 the timed solve is its Cholesky
+    solver = O.SOLVE_PINV if FI.shape[1] <= 126 else O.SOLVE_CHOL
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    dt, note = _oracle_rate_at(O, T, FI, Y0, b0, ns, threads, solver)
+    # GPU paths on the same sample, one iteration from the same state
+    m1 = O.Model(np.ascontiguousarray(T[:ns]), FI, Y0, b0, lam=LAMBDA, solver=O.SOLVE_CHOL)
+    llh_o, _, _ = m1.estep(1, TEMP)
+    m1.mstep()
+    beta_o = m1.beta
+    del m1
+    vs = {}
+    for name, prec in (("fp64", dg.FP64), ("mixed", dg.MIXED)):
+        with dg.Engine(precision=prec, lam=LAMBDA) as e:
+            e.upload(T[:ns], FI, Y0, b0)
+            llh, _, _ = e.estep(True, TEMP)
+            beta = e.mstep()
+        vs[name] = {"llh_rel_err": abs(llh - llh_o) / abs(llh_o),
+                    "beta_rel_err": abs(beta - beta_o) / beta_o}
+    single = None
+    if ns1 != 0:
+        ns1 = N if ns1 is None else min(ns1, N)
+        dt1, note1 = _oracle_rate_at(O, T, FI, Y0, b0, ns1, 1, solver)
+        single = {"value": 1.0 / dt1, "unit": "iter/s", "sample": note1}
+    return {"value": 1.0 / dt, "unit": "iter/s", "cores": threads, "kind": "port",
+            "sample": f"oracle/liboracle.so (FP64, reference step order, materialised D and R, "
+                      f"{'pseudo-inverse' if solver == O.SOLVE_PINV else 'Cholesky'} solve) "
+                      f"with {threads} threads on {note}; K={FI.shape[0]}, D={T.shape[1]}",
+            "nproc": os.cpu_count(), "threads_allowed": len(os.sched_getaffinity(0)),
+            "single_thread": single,
+            "gpu_vs_oracle_on_sample": {"points": ns, **vs}}
+
+
+def device_peaks():
+    with dg.Engine(precision=dg.FP64) as e:
+        return e.device_peaks()
 
 
 def c4_instances(n=64, seed=2024):
@@ -186,10 +273,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--fp64-steps", type=int, default=3)
-    ap.add_argument("--N", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=16000)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="c3",
+                    help="c3 (default): the headline workload; c1, c2, c5: the other "
+                         "single-instance configurations (c4 is --batch)")
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["mixed", "mixed_r1"], default="mixed",
                     help="mixed: R as an FP16 hi/lo pair; mixed_r1: R as one FP16 term")
@@ -222,9 +310,13 @@ def main():
         batch_bench(args)
         return
 
-    N, D, K, M, L = args.N, 166, 8000, 125, 3
-    print(f"[bench] generating N={N} D={D} K={K} M+1={M + 1}", file=sys.stderr, flush=True)
-    T, X, FI, Y0, b0 = build_problem(N, D, K, M, L, seed=7)
+    cfg = args.config
+    N, D, K, M, L, data, desc = CONFIGS[cfg]
+    if args.N:
+        N = args.N
+    fp64_steps = args.fp64_steps or (1 if cfg == "c5" else 3)
+    print(f"[bench] {cfg}: generating N={N} D={D} K={K} M+1={M + 1}", file=sys.stderr, flush=True)
+    T, X, FI, Y0, b0 = build_problem(N, D, K, M, L, data)
 
     # ---- mixed precision (the measured path)
     prec16 = dg.MIXED_R1 if args.precision == "mixed_r1" else dg.MIXED
@@ -245,6 +337,18 @@ def main():
     it16 = args.steps / sec16
     e16.close()
 
+    # the same upload from rows padded as a gsl_matrix pads them (64 bytes)
+    upload_padded_s = None
+    if cfg in ("c3", "c5"):
+        ldT = (D * 8 + 63) // 64 * 8
+        Tp = np.zeros((N, ldT))
+        Tp[:, :D] = T
+        with dg.Engine(precision=prec16, lam=LAMBDA) as ep:
+            t_up = time.perf_counter()
+            ep.upload_strided(Tp, D, FI, Y0, b0)
+            upload_padded_s = time.perf_counter() - t_up
+        del Tp
+
     # ---- FP64 GPU path from the same initial state
     e64 = dg.Engine(precision=dg.FP64, lam=LAMBDA)
     e64.upload(T, FI, Y0, b0)
@@ -252,9 +356,9 @@ def main():
     print("[bench] fp64", file=sys.stderr, flush=True)
     _, llh64_w, _ = run_steps(e64, 1)
     st64 = e64.download()
-    sec64, llh64_t, ph64 = run_steps(e64, args.fp64_steps)
+    sec64, llh64_t, ph64 = run_steps(e64, fp64_steps)
     llh64 = llh64_w + llh64_t
-    it64 = args.fp64_steps / sec64
+    it64 = fp64_steps / sec64
     e64.close()
 
     n = min(len(llh16), len(llh64))
@@ -274,35 +378,58 @@ def main():
     achieved = flop / (dom_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP16_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 4),
-            "traffic": measured_traffic(dom),
+            "traffic": measured_traffic(dom) if cfg == "c3" else None,
             "kernel": dom, "kernel_desc": name, "kernel_ms": round(dom_ms, 4),
             "algorithmic_flop_per_launch": flop}
+
+    # ---- step roofline: t_lb = max(F / MFMA peak, E / exp peak, B / HBM peak)
+    # with the MFMA and exponential peaks measured on this device
+    print("[bench] device peaks", file=sys.stderr, flush=True)
+    pk_mfma, pk_exp, pk_f64 = device_peaks()
+    bT = 2.0 if data != "gmm" else 4.0      # stored T width: FP16, or an FP16 hi/lo pair
+    F_alg, E_alg = 4.0 * K * N * D, 3.0 * K * N
+    B_alg = N * D * bT + 16.0 * K * D + 8.0 * N
+    t_lb = {"mfma": F_alg / (pk_mfma * 1e12), "exp": E_alg / pk_exp,
+            "hbm": B_alg / (PEAK_HBM_GBPS * 1e9)}
+    bind = max(t_lb, key=t_lb.get)
+    t_step = 1.0 / it16
+    step_roof = {"t_lb_ms": round(1e3 * t_lb[bind], 4), "binding": bind,
+                 "frac": round(t_lb[bind] / t_step, 4),
+                 "t_lb_ms_each": {k: round(1e3 * v, 4) for k, v in t_lb.items()},
+                 "measured_peaks": {"mfma_fp16_tflops": round(pk_mfma, 1),
+                                    "exp_per_s": pk_exp, "fp64_fma_tflops": round(pk_f64, 2)},
+                 "F_alg_flop": F_alg, "E_alg_exp": E_alg, "B_alg_bytes": B_alg}
+    fp64_frac = F_alg * it64 / (PEAK_FP64_TFLOPS * 1e12)
 
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(T, FI, Y0, b0, min(args.cpu_sample, N), args.cpu_iters)
+        cpu = cpu_baseline(cfg, T, FI, Y0, b0)
 
     out = {
-        "metric": METRIC, "value": round(it16, 3), "unit": "iter/s", "n_gpus": 1,
+        "metric": METRIC if cfg == "c3" else
+        f"DA-GTM E+M iter/s at N={N},K={K},D={D} ({cfg}): mixed-prec vs FP64 GPU",
+        "value": round(it16, 3), "unit": "iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 / it16, 3),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": ("fp16x2-mfma+fp32 (E-step), fp64 (M-step)" if prec16 == dg.MIXED else
                   "fp16x2 distances, fp16 R, fp32 accumulation (E-step), fp64 (M-step)"),
         "data": "synthetic",
-        "config": {"workload": "N=1,000,000 binary fingerprints D=166 (50 prototypes, "
-                               "flip 0.05, seed 7), L=3 K=8000, M+1=126, s=1, lambda=1e-3, "
-                               "DA temp 2.0; one step = one E+M iteration",
-                   "N": N, "D": D, "K": K, "M1": M + 1, "parallelism": "single GPU"},
+        "config": {"workload": desc if args.N is None else desc + f" (N cut to {N})",
+                   "name": cfg, "N": N, "D": D, "K": K, "M1": M + 1,
+                   "parallelism": "single GPU"},
         "fp64_gpu_iter_per_s": round(it64, 4),
         "mixed_over_fp64": round(it16 / it64, 2),
+        "fp64_path_frac_of_fp64_peak": round(fp64_frac, 4),
         "llh_rel_err_vs_fp64": llh_err,
         "state_rel_err_vs_fp64_after_1_iter": state_err,
         "kd_updates_per_s": K * N * D * it16,
         "phase_ms_mixed": [round(x / args.steps, 4) for x in ph16],
         "upload_s": round(upload_s, 3),
-        "phase_ms_fp64": [round(x / args.fp64_steps, 4) for x in ph64],
+        "upload_padded_rows_s": None if upload_padded_s is None else round(upload_padded_s, 3),
+        "phase_ms_fp64": [round(x / fp64_steps, 4) for x in ph64],
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

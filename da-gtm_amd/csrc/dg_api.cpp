@@ -855,6 +855,21 @@ int dg_get_stats(dg_ctx *c, double *G, double *RT, long ldRT) {
     return DG_OK;
 }
 
+int dg_device_peaks(dg_ctx *c, double *mfma_f16_tflops, double *exp_per_s, double *fp64_tflops) {
+    if (!c) return DG_EINVAL;
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    int cus = 0;
+    DG_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    double m = 0, e = 0, f = 0;
+    // eight 4-wave workgroups per CU: eight waves per SIMD
+    if ((rc = dg_peaks_run(c, 8 * std::max(cus, 1), &m, &e, &f))) return rc;
+    if (mfma_f16_tflops) *mfma_f16_tflops = m;
+    if (exp_per_s) *exp_per_s = e;
+    if (fp64_tflops) *fp64_tflops = f;
+    return DG_OK;
+}
+
 int dg_set_timing(dg_ctx *c, int on) {
     if (!c) return DG_EINVAL;
     c->timing = on ? 1 : 0;

@@ -1,38 +1,152 @@
-// The annealing control of dagtm.cxx:1508-2206 over caller-supplied steps
-// (dg_anneal_ctl in dagtm_gpu.h).  Host code only: no device calls here, so
-// the control can be exercised with scripted E-steps on any machine.
+// The annealing control of dagtm.cxx:1508-2206: the resumable machine of
+// dg_control.h and dg_anneal_ctl (dagtm_gpu.h), which drives one machine with
+// blocking callbacks.  Host code only: no device calls here, so the control
+// can be exercised with scripted E-steps on any machine.
+#include "dg_control.h"
+
 #include <cfloat>
 
-#include "../../include/dagtm_gpu.h"
+namespace dgc {
 
 namespace {
+inline double absv(double a) { return (a > 0) ? a : -a; }
+}  // namespace
 
-// The reference's 50-entry moving-average queue (dagtm.cxx:45, 1244, 1521;
+AnnealMachine::AnnealMachine(const dg_anneal_params &p, dg_hda_state *hda)
+    : p_(p), hda_(hda),
+      use_hda_(hda && hda->enable && p.isDA && p.schedule == DG_SCHED_EXP) {
+    if (hda_) hda_->nexpanded = 0;
+    temp_ = p_.isDA ? p_.start_temp : 1.0;
+    nloop_ = p_.nloop0;
+    begin_temperature();
+}
+
+// The reference's 50-entry moving-average queue (dagtm.cxx:1244, 1521;
 // comm.c:1704-1746): refilled with DBL_MAX per temperature, so its mean is
 // +inf (or NaN after a difference) until at most one refill value is left,
 // and the sum runs in index order, as dagtm_vector_sum does (comm.c:208-218).
-struct QualQueue {
-    static constexpr int kSize = 50;
-    double v[kSize];
-    int next = 0;
-    void reset() {
-        for (double &x : v) x = DBL_MAX;
-        next = 0;
-    }
-    void push(double x) {
-        v[next] = x;
-        next = (next + 1) % kSize;
-    }
-    double mean() const {
-        double s = 0.0;
-        for (double x : v) s += x;
-        return s / kSize;
-    }
-};
+void AnnealMachine::queue_reset() {
+    for (double &x : q_) x = DBL_MAX;
+    qnext_ = 0;
+}
 
-inline double absv(double a) { return (a > 0) ? a : -a; }
+double AnnealMachine::queue_mean() const {
+    double s = 0.0;
+    for (double x : q_) s += x;
+    return s / kQ;
+}
 
-}  // namespace
+// dagtm.cxx:1510-1530: clamp, refill the queue, remember the llh, run.
+void AnnealMachine::begin_temperature() {
+    if (temp_ < 1.0) { temp_ = 1.0; isLast_ = 1; }
+    queue_reset();
+    ntemps_++;
+    qual_before_temp_ = llh_;
+    lloop_ = 0;
+    act_ = ESTEP;
+}
+
+// dagtm.cxx:1553-1603: the qualities of this iteration
+void AnnealMachine::estep_done(double llh, double nfree) {
+    llh_ = llh;
+    nfree_ = nfree;
+    qual_old_ = qual_;
+    lloop_++;
+    nloop_++;
+    qual_ = p_.isDA ? nfree_ : llh_;
+    diff_ = qual_ - qual_old_;
+}
+
+// dagtm.cxx:1689-1709: both convergence tests, before the M-step
+void AnnealMachine::decide() {
+    if (absv(diff_) < p_.eps) { end_temperature(); return; }
+    if (nloop_ > kQ) {
+        const double before = queue_mean();
+        q_[qnext_] = qual_;
+        qnext_ = (qnext_ + 1) % kQ;
+        if (absv(queue_mean() - before) < p_.eps) { end_temperature(); return; }
+    }
+    act_ = MSTEP;
+}
+
+// dagtm.cxx:1790-1795
+void AnnealMachine::mstep_done() {
+    if (p_.maxloop && nloop_ >= p_.maxloop) {
+        isLast_ = 1;
+        exit_maxloop_ = 1;
+        end_temperature();
+        return;
+    }
+    act_ = ESTEP;
+}
+
+// dagtm.cxx:1909-2179: the next temperature, or the end of the run.
+void AnnealMachine::end_temperature() {
+    if (!p_.isDA || isLast_) { act_ = DONE; return; }
+    if (p_.schedule == DG_SCHED_LINEAR) {
+        temp_ -= p_.delta;
+        begin_temperature();
+    } else if (p_.schedule == DG_SCHED_AUTO) {
+        ctemp_for_ = 0;
+        act_ = CTEMP;
+    } else if (use_hda_ && hda_->K < hda_->Kmax) {
+        // HDA (1938-1977): an armed expansion fires on enough llh gain and
+        // is cancelled otherwise; an unarmed state searches the next critical
+        // temperature and arms when it is within one cooling step
+        const double gain = llh_ - qual_before_temp_;
+        fire_ = false;
+        if (ready_) {
+            if (gain < hda_->eps2) ready_ = false;
+            else fire_ = true;
+        }
+        if (!ready_) {
+            ctemp_for_ = 1;
+            act_ = CTEMP;
+            return;
+        }
+        cool_exp();
+    } else {
+        temp_ *= p_.alpha;
+        begin_temperature();
+    }
+}
+
+// the rest of the EXP branch after the arming search: fire, then cool
+void AnnealMachine::cool_exp() {
+    if (fire_) {
+        act_ = EXPAND;
+        return;
+    }
+    temp_ *= p_.alpha;
+    begin_temperature();
+}
+
+void AnnealMachine::ctemp_done(double ct) {
+    if (ctemp_for_ == 0) {                 // AUTO, dagtm.cxx:1918-1934
+        temp_ = (ct >= temp_) ? temp_ * p_.alpha : ct;
+        begin_temperature();
+        return;
+    }
+    if (ct >= temp_ * p_.alpha) ready_ = true;
+    cool_exp();
+}
+
+void AnnealMachine::expand_done(int K) {
+    ready_ = false;
+    fire_ = false;
+    hda_->K = K;
+    hda_->nexpanded++;
+    temp_ *= p_.alpha;
+    begin_temperature();
+}
+
+void AnnealMachine::result(dg_anneal_result *r) const {
+    r->nloop = nloop_; r->ntemps = ntemps_; r->exit_maxloop = exit_maxloop_;
+    r->temp = temp_; r->llh = llh_; r->nfree = nfree_; r->qual = qual_;
+    r->beta = 0.0;                         // the step provider owns beta
+}
+
+}  // namespace dgc
 
 extern "C" int dg_anneal_ctl(const dg_anneal_params *p, dg_hda_state *hda,
                              const dg_anneal_ops *ops, void *user, dg_anneal_result *res) {
@@ -44,72 +158,42 @@ extern "C" int dg_anneal_ctl(const dg_anneal_params *p, dg_hda_state *hda,
     if ((p->isDA && p->schedule == DG_SCHED_AUTO && !ops->next_ctemp) ||
         (use_hda && (!ops->next_ctemp || !ops->expand)))
         return DG_EINVAL;
-    if (hda) hda->nexpanded = 0;
-
-    QualQueue Q;
-    double temp = p->isDA ? p->start_temp : 1.0;
-    int isLast = 0, nloop = p->nloop0, ntemps = 0, exit_maxloop = 0, rc = DG_OK;
-    double qual = 0.0, qual_old = 0.0, llh = 0.0, nfree = 0.0;
-    bool ready = false;                      // bReadyToExpand, dagtm.cxx:1504
-    auto finish = [&](int status) {
-        res->nloop = nloop; res->ntemps = ntemps; res->exit_maxloop = exit_maxloop;
-        res->temp = temp; res->llh = llh; res->nfree = nfree; res->qual = qual;
-        res->beta = 0.0;                     // the step provider owns beta
-        return status;
-    };
-    do {
-        if (temp < 1.0) { temp = 1.0; isLast = 1; }
-        Q.reset();
-        ntemps++;
-        const double qual_before_temp = llh;
-        int lloop = 0;
-        while (true) {
-            if ((rc = ops->estep(user, p->isDA, temp, &llh, &nfree))) return finish(rc);
-            qual_old = qual;
-            lloop++; nloop++;
-            qual = p->isDA ? nfree : llh;
-            const double diff = qual - qual_old;
-            if (ops->on_iter && (rc = ops->on_iter(user, nloop, lloop, temp, nfree, llh, diff)))
-                return finish(rc);
-            if (absv(diff) < p->eps) break;
-            if (nloop > QualQueue::kSize) {
-                const double before = Q.mean();
-                Q.push(qual);
-                if (absv(Q.mean() - before) < p->eps) break;
+    dgc::AnnealMachine m(*p, hda);
+    int rc = DG_OK;
+    while (rc == DG_OK && m.next() != dgc::AnnealMachine::DONE) {
+        switch (m.next()) {
+            case dgc::AnnealMachine::ESTEP: {
+                double llh = 0.0, nfree = 0.0;
+                const double t = m.temp();
+                if ((rc = ops->estep(user, p->isDA, t, &llh, &nfree))) break;
+                m.estep_done(llh, nfree);
+                // the progress row and checkpoint come before the tests
+                if (ops->on_iter &&
+                    (rc = ops->on_iter(user, m.nloop(), m.lloop(), t, nfree, llh, m.diff())))
+                    break;
+                m.decide();
+                break;
             }
-            if ((rc = ops->mstep(user))) return finish(rc);
-            if (p->maxloop && nloop >= p->maxloop) { isLast = 1; exit_maxloop = 1; break; }
-        }
-        if (!p->isDA || isLast) break;
-        if (p->schedule == DG_SCHED_LINEAR) {
-            temp -= p->delta;
-        } else if (p->schedule == DG_SCHED_AUTO) {
-            double ct = 0.0;
-            if ((rc = ops->next_ctemp(user, temp, p->eps, &ct))) return finish(rc);
-            temp = (ct >= temp) ? temp * p->alpha : ct;
-        } else {
-            if (use_hda && hda->K < hda->Kmax) {
-                const double gain = llh - qual_before_temp;
-                bool fire = false;
-                if (ready) {
-                    if (gain < hda->eps2) ready = false;
-                    else fire = true;
-                }
-                if (!ready) {
-                    double ct = 0.0;
-                    if ((rc = ops->next_ctemp(user, temp, p->eps, &ct))) return finish(rc);
-                    if (ct >= temp * p->alpha) ready = true;
-                }
-                if (fire) {
-                    ready = false;
-                    int K = hda->K;
-                    if ((rc = ops->expand(user, nloop, temp, &K))) return finish(rc);
-                    hda->K = K;
-                    hda->nexpanded++;
-                }
+            case dgc::AnnealMachine::MSTEP:
+                if ((rc = ops->mstep(user))) break;
+                m.mstep_done();
+                break;
+            case dgc::AnnealMachine::CTEMP: {
+                double ct = 0.0;
+                if ((rc = ops->next_ctemp(user, m.temp(), p->eps, &ct))) break;
+                m.ctemp_done(ct);
+                break;
             }
-            temp *= p->alpha;
+            case dgc::AnnealMachine::EXPAND: {
+                int K = hda->K;
+                if ((rc = ops->expand(user, m.nloop(), m.temp(), &K))) break;
+                m.expand_done(K);
+                break;
+            }
+            case dgc::AnnealMachine::DONE:
+                break;
         }
-    } while (!isLast);
-    return finish(DG_OK);
+    }
+    m.result(res);
+    return rc;
 }

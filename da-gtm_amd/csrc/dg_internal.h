@@ -95,6 +95,30 @@ enum {
     kScalCount = 8
 };
 
+// ---- batched instances ----------------------------------------------------
+// One instance of a batched step, as the batched kernels read it from a
+// device table: the shape, the device buffers of its context and the
+// parameters of the current step.  Built on the host from each context.
+struct BInst {
+    // shape
+    long N, Np, CL;
+    int D, K, Kp, M1, ldt, ldh, NC, nchAB, KS, TT;
+    // FP64 state (dg_ctx)
+    const double *T64, *tt, *FI;
+    double *Y64, *yy, *W, *A, *A0, *Bm, *G, *Gm, *RT, *pm, *pl, *logg, *logh, *partAB, *scal;
+    // mixed state
+    const _Float16 *T16, *T16t;
+    _Float16 *Y16;
+    float *tt32, *yy32, *aK32, *pP32, *blkThr, *partRT32, *partG32;
+    double *mscal, *partLog, *amaxPart;
+    // step parameters
+    float beta_f, it;
+    int isDA, two, do_e, do_m;
+    double logK, ridge;
+    // first work item of this instance in its pass-1 / pass-2 launch
+    int item_e1, item_e2;
+};
+
 int dg_fail(dg_ctx *ctx, int code, const std::string &msg);
 int dg_hip_check(dg_ctx *ctx, hipError_t e, const char *what);
 
@@ -131,7 +155,9 @@ int dg_mstep_run(dg_ctx *ctx);
 // Y = FI W and |y_k|^2 from the context's FI and W.
 int dg_centres_from_w(dg_ctx *ctx);
 int dg_rownorms(dg_ctx *ctx, const double *X, long rows, int ld, int D, double *out);
-// dst (rows x ld, zero padded) from the dense rows x D src, both on the device.
+// timed dependency-chain kernels (dg_peaks.hip): FP16 MFMA, v_exp_f32, FP64 FMA
+int dg_peaks_run(dg_ctx *ctx, int wgs, double *mfma_tflops, double *exp_per_s,
+                 double *f64_tflops);
 // dst[r][d] = src[r lds + d] for d < D, 0 for D <= d < ld (device arrays)
 int dg_pad_rows(dg_ctx *ctx, const double *src, long rows, int D, long lds, int ld, double *dst);
 int dg_reduce_logs(dg_ctx *ctx);

@@ -89,10 +89,12 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 // registers and runs three) makes the compiler keep the accumulator in
 // ordinary VGPRs, where the exponent FMAs read it directly; in AGPRs every
 // element would first need a v_accvgpr_read.
+// The body of one workgroup (point tile bx); k_e1_mfma runs one per
+// workgroup, kb_e1_mfma takes tiles of many instances from a work queue.
 template <int KS, int TT, bool TWO>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k_e1_mfma(
-    const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
-    const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
+__device__ __forceinline__ void e1_body(
+    const long bx, const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N,
+    long Np, const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
     double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
 
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    const long n = (long)blockIdx.x * PT + 32 * w + j;
+    const long n = bx * PT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
     // exponents are carried without this point's constant bn = cb |t_n|^2:
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     float bt = ptv;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) bt = fmaxf(bt, __shfl_xor(bt, o));
-    if (l == 0) blkThr[(long)blockIdx.x * (PT / BP) + w] = bt;
+    if (l == 0) blkThr[bx * (PT / BP) + w] = bt;
     // workgroup partials of sum log g, sum log h and the zero-column count
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) {
@@ -376,8 +378,19 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (l == 0) { wsum[0][w] = lg; wsum[1][w] = lh; wsum[2][w] = zc; }
     __syncthreads();
     if (tid < 3)
-        partLog[(long)blockIdx.x * 3 + tid] =
+        partLog[bx * 3 + tid] =
             (wsum[tid][0] + wsum[tid][1]) + (wsum[tid][2] + wsum[tid][3]);
+}
+
+template <int KS, int TT, bool TWO>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k_e1_mfma(
+    const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
+    const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
+    const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
+    float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
+    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
+    e1_body<KS, TT, TWO>(blockIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal, beta, it, isDA, logK,
+                         pP, blkThr, pm, logg, logh, partLog);
 }
 
 // scal[0..2] = sums over the pass-1 workgroup partials (fixed tree order).
@@ -432,9 +445,12 @@ __global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float b
 // that row.  The 1-D grid is mapped so that the workgroups the hardware places
 // on one XCD (linear id mod 8) share a point chunk and walk all its centre
 // groups, keeping that chunk's data in the XCD's L2.
+// The body of one workgroup (linear index L: a centre group and a point
+// chunk); k_e2_mfma runs one per workgroup, kb_e2_mfma takes them from a
+// work queue over many instances.
 template <int KS, int TT, bool GM, bool RLO>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_e2_mfma(
-    const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
+__device__ __forceinline__ void e2_body(
+    const int L, const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
@@ -459,7 +475,6 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     static_assert(NS * SLOT * 2 <= 160 * 1024, "LDS ring");
     __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
 
-    const int L = blockIdx.x;
     const int xs = L & 7, sl = L >> 3;
     const int kg = sl % KG;
     const long cidx = (long)(sl / KG) * 8 + xs;
@@ -811,6 +826,17 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const float Go = __shfl_xor(G, 32), Gco = __shfl_xor(Gc, 32);
         if (h == 0 && k < K) partG[cidx * K + k] = (((G - Gc) + (Go - Gco))) * ginv;
     }
+}
+
+template <int KS, int TT, bool GM, bool RLO>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_e2_mfma(
+    const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
+    const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
+    const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
+    const float *__restrict__ blkThr, long chunk, int KG,
+    float *__restrict__ partRT, float *__restrict__ partG) {
+    e2_body<KS, TT, GM, RLO>(blockIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D, mscal, beta, it, pP,
+                             blkThr, chunk, KG, partRT, partG);
 }
 
 // ---------------------------------------------------------------- helpers

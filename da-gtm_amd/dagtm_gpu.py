@@ -21,7 +21,8 @@ SCHED_EXP, SCHED_LINEAR, SCHED_AUTO = 0, 1, 2
 EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload",
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
-           "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl", "dg_map_noalloc"]
+           "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl", "dg_map_noalloc",
+           "dg_device_peaks"]
 
 
 class DgConfig(C.Structure):
@@ -111,6 +112,8 @@ def load():
     L.dg_next_ctemp.argtypes = [vp, C.c_double, C.c_double, vp, C.POINTER(C.c_double), vp]
     L.dg_map.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
     L.dg_map_noalloc.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
+    L.dg_device_peaks.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
     L.dg_resp_dense.argtypes = [vp, _dp, C.c_long]
     L.dg_download.argtypes = [vp, vp, C.c_long, vp, C.c_long, C.POINTER(C.c_double)]
     L.dg_get_stats.argtypes = [vp, vp, vp, C.c_long]
@@ -225,6 +228,15 @@ class Engine:
         self._chk(self.L.dg_upload(self.h, T, N, D, D, FI, K, M1, M1, Y, D, float(beta)))
         self.N, self.D, self.K, self.M1 = N, D, K, M1
 
+    def upload_strided(self, Tp, D, FI, Y, beta):
+        """upload() from data rows with a row stride: Tp is N x ldT with the
+        data in its first D columns (a gsl_matrix's tda layout)."""
+        Tp, FI, Y = _f64(Tp), _f64(FI), _f64(Y)
+        N, ldT = Tp.shape
+        K, M1 = FI.shape
+        self._chk(self.L.dg_upload(self.h, Tp, N, D, ldT, FI, K, M1, M1, Y, D, float(beta)))
+        self.N, self.D, self.K, self.M1 = N, D, K, M1
+
     def set_state(self, Y, beta):
         Y = _f64(Y)
         self._chk(self.L.dg_set_state(self.h, Y, Y.shape[1], float(beta)))
@@ -313,6 +325,12 @@ class Engine:
         G = np.zeros(self.K); RT = np.zeros((self.K, self.D))
         self._chk(self.L.dg_get_stats(self.h, G.ctypes.data, RT.ctypes.data, self.D))
         return G, RT
+
+    def device_peaks(self):
+        """Measured (FP16 MFMA TFLOP/s, v_exp_f32 per s, FP64 FMA TFLOP/s)."""
+        m, e, f = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.dg_device_peaks(self.h, C.byref(m), C.byref(e), C.byref(f)))
+        return m.value, e.value, f.value
 
     def set_timing(self, on=True):
         self._chk(self.L.dg_set_timing(self.h, int(on)))

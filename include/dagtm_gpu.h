@@ -279,6 +279,14 @@ int dg_download(dg_ctx *ctx, double *W, long ldW, double *Y, long ldY,
 /* Copies G (K) and RT (K x D, ldRT >= D) of the last E-step (testing). */
 int dg_get_stats(dg_ctx *ctx, double *G, double *RT, long ldRT);
 
+/* Measured peaks of the context's device (no counterpart in the reference;
+ * the denominators of roofline fractions): FP16 MFMA TFLOP/s
+ * (v_mfma_f32_32x32x16_f16 chains), transcendental exponentials per second
+ * (v_exp_f32) and vector FP64 FMA TFLOP/s, each from one timed launch of a
+ * dependency-chain kernel on every CU.  Any output pointer may be NULL. */
+int dg_device_peaks(dg_ctx *ctx, double *mfma_f16_tflops, double *exp_per_s,
+                    double *fp64_tflops);
+
 /* Device time of the last dg_estep / dg_mstep, per kernel phase, in ms:
  * t[0] E-pass 1, t[1] E-pass 2, t[2] reductions, t[3] M-step.  Timing is
  * recorded only after dg_set_timing(ctx, 1). */

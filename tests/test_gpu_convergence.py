@@ -1,0 +1,182 @@
+"""Whole annealing runs to convergence on the N=1000, D=12 Gaussian-mixture
+configuration (K = 400, L = 2, M + 1 = 26, s = 1, lambda = 1e-3, EXP cooling
+alpha = 0.99 from the first critical temperature / alpha^10, eps = 1e-6, no
+iteration cap): 192 temperatures, several thousand E+M iterations.
+
+  - FP64 GPU vs the oracle: the same number of iterations and temperatures
+    (every |diff| and window decision agrees), per-iteration llh, -F and beta
+    within 1e-8, the final centres within 1e-9.
+  - Mixed precision shadowing FP64: the mixed engine takes exactly the steps
+    the FP64 run's decisions call for (dg_anneal_ctl driven by the FP64
+    qualities), and its state is compared after every M-step.  Stated
+    tolerances over the whole run: llh 5e-6, -F and beta 2e-4, W 2e-4, Y 1e-4
+    (the largest differences occur at the phase transitions, where the DA
+    dynamics amplify them, and shrink again); at the end of the run llh,
+    beta, W and Y within 5e-6.
+  - Mixed running its own decisions does not take the same number of
+    iterations: its per-iteration quality carries errors of about 3e-7
+    relative (4e-6 absolute at |qual| ~ 12.5), above eps = 1e-6 absolute, so
+    some |diff| < eps decisions differ.  It visits the same temperatures and
+    ends within 5e-6 of the FP64 llh.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dagtm_gpu as dg
+import datagen
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ALPHA, EPS, LAM = 0.99, 1e-6, 1e-3
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    T = datagen.gmm(1000, 12, 42)
+    X, FI = datagen.latent_basis(2, 400, 25, 1.0)
+    W0, Y0, b0 = datagen.pca_init(T, FI, X)
+    t0 = O.ctemp(T) / ALPHA ** 10
+    return T, FI, Y0, b0, t0
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_fp64_full_anneal_matches_oracle(c1):
+    T, FI, Y0, b0, t0 = c1
+    _say("case: full anneal c1, FP64 GPU vs oracle")
+    O.set_threads(16)
+    m = O.Model(T, FI, Y0, b0, lam=LAM, solver=O.SOLVE_CHOL)
+    ro, rows_o = m.anneal(isDA=True, alpha=ALPHA, eps=EPS, start_temp=t0)
+    with dg.Engine(precision=dg.FP64, lam=LAM) as e:
+        e.upload(T, FI, Y0, b0)
+        rg, rows_g = e.anneal(isDA=True, alpha=ALPHA, eps=EPS, start_temp=t0)
+        _, Y, beta = e.download()
+    _say(f"  nloop gpu {rg.nloop} oracle {ro.nloop}, ntemps {rg.ntemps} / {ro.ntemps}")
+    assert rg.nloop == ro.nloop and rg.ntemps == ro.ntemps and ro.ntemps > 100
+    assert not rg.exit_maxloop and not ro.exit_maxloop
+    a, b = np.array(rows_g), np.array(rows_o)
+    assert np.array_equal(a[:, 2], b[:, 2])                            # temperatures
+    for col in (3, 4, 6):                                              # -F, llh, beta
+        assert np.allclose(a[:, col], b[:, col], rtol=1e-8, atol=1e-10), col
+    assert rg.llh == pytest.approx(ro.llh, rel=1e-10)
+    assert _rel(Y, m.Y) < 1e-9 and beta == pytest.approx(m.beta, rel=1e-9)
+
+
+class _Shadow:
+    """FP64 engine drives the control; the mixed engine takes the same steps."""
+
+    def __init__(self, e64, e16):
+        self.a, self.b = e64, e16
+        self.rows = []
+
+    def estep(self, isDA, temp):
+        la, fa, _ = self.a.estep(isDA, temp)
+        lb, fb, _ = self.b.estep(isDA, temp)
+        self.cur = [temp, la, lb, fa, fb]
+        return la, fa
+
+    def mstep(self):
+        ba, bb = self.a.mstep(), self.b.mstep()
+        Wa, Ya, _ = self.a.download()
+        Wb, Yb, _ = self.b.download()
+        self.rows.append(self.cur + [ba, bb, _rel(Wb, Wa), _rel(Yb, Ya)])
+
+
+def test_mixed_shadowing_fp64_over_full_anneal(c1):
+    T, FI, Y0, b0, t0 = c1
+    _say("case: full anneal c1, mixed shadowing FP64 decisions")
+    with dg.Engine(precision=dg.FP64, lam=LAM) as a, dg.Engine(precision=dg.MIXED, lam=LAM) as b:
+        a.upload(T, FI, Y0, b0)
+        b.upload(T, FI, Y0, b0)
+        s = _Shadow(a, b)
+        rc, r, _, _ = dg.anneal_ctl(s, isDA=True, alpha=ALPHA, eps=EPS, start_temp=t0)
+        assert rc == 0
+    R = np.array(s.rows)
+    llh = np.abs(R[:, 2] - R[:, 1]) / np.abs(R[:, 1])
+    nfree = np.abs(R[:, 4] - R[:, 3]) / np.abs(R[:, 3])
+    beta = np.abs(R[:, 6] - R[:, 5]) / R[:, 5]
+    _say(f"  {r.nloop} iterations; max rel err llh {llh.max():.2e} (median {np.median(llh):.2e}) "
+         f"-F {nfree.max():.2e} beta {beta.max():.2e} W {R[:, 7].max():.2e} Y {R[:, 8].max():.2e}; "
+         f"end llh {llh[-1]:.2e} beta {beta[-1]:.2e} W {R[-1, 7]:.2e} Y {R[-1, 8]:.2e}")
+    assert r.nloop > 1000
+    assert llh.max() < 5e-6 and nfree.max() < 2e-4 and beta.max() < 2e-4
+    assert R[:, 7].max() < 2e-4 and R[:, 8].max() < 1e-4
+    assert max(llh[-1], beta[-1], R[-1, 7], R[-1, 8]) < 5e-6
+
+
+def test_mixed_free_running_anneal(c1):
+    T, FI, Y0, b0, t0 = c1
+    _say("case: full anneal c1, mixed on its own decisions")
+    out = {}
+    for prec in (dg.FP64, dg.MIXED):
+        with dg.Engine(precision=prec, lam=LAM) as e:
+            e.upload(T, FI, Y0, b0)
+            out[prec], _ = e.anneal(isDA=True, alpha=ALPHA, eps=EPS, start_temp=t0)
+    r64, r16 = out[dg.FP64], out[dg.MIXED]
+    _say(f"  nloop fp64 {r64.nloop} mixed {r16.nloop}; final llh {r64.llh!r} {r16.llh!r}")
+    assert r16.ntemps == r64.ntemps
+    assert r16.llh == pytest.approx(r64.llh, rel=5e-6)
+
+
+class _ShadowSparse(_Shadow):
+    """As _Shadow, comparing W and Y every `every` M-steps."""
+
+    def __init__(self, e64, e16, every):
+        super().__init__(e64, e16)
+        self.every, self.n = every, 0
+
+    def mstep(self):
+        ba, bb = self.a.mstep(), self.b.mstep()
+        self.n += 1
+        row = self.cur + [ba, bb, np.nan, np.nan]
+        if self.n % self.every == 0:
+            Wa, Ya, _ = self.a.download()
+            Wb, Yb, _ = self.b.download()
+            row[7], row[8] = _rel(Wb, Wa), _rel(Yb, Ya)
+        if self.n % 1000 == 0:
+            _say(f"  step {self.n} temp {self.cur[0]:.4f}")
+        self.rows.append(row)
+
+
+def test_mixed_shadowing_fp64_fingerprints():
+    """The same on the K = 8000, D = 166 fingerprint configuration at
+    N = 5000 (about 5300 iterations, 189 temperatures).  Across its phase
+    transitions the centres of the two runs move apart transiently (W, Y up
+    to ~1e-2 relative) and come back together: stated tolerances llh 2e-4,
+    -F and beta 2e-4 over the run, W 5e-2 and Y 3e-2 (compared every 25
+    M-steps); at the end llh and beta 5e-6, W and Y 5e-5."""
+    T = datagen.fingerprints(100_000, 166, 7)[:5000].astype(np.float64)
+    X, FI = datagen.latent_basis(3, 8000, 125, 1.0)
+    W0, Y0, b0 = datagen.pca_init(T, FI, X)
+    t0 = O.ctemp(T) / ALPHA ** 10
+    _say("case: full anneal, fingerprints N=5000 K=8000, mixed shadowing FP64 decisions")
+    with dg.Engine(precision=dg.FP64, lam=LAM) as a, dg.Engine(precision=dg.MIXED, lam=LAM) as b:
+        a.upload(T, FI, Y0, b0)
+        b.upload(T, FI, Y0, b0)
+        s = _ShadowSparse(a, b, 25)
+        rc, r, _, _ = dg.anneal_ctl(s, isDA=True, alpha=ALPHA, eps=EPS, start_temp=t0)
+        assert rc == 0
+        Wa, Ya, _ = a.download()
+        Wb, Yb, _ = b.download()
+    R = np.array(s.rows)
+    llh = np.abs(R[:, 2] - R[:, 1]) / np.abs(R[:, 1])
+    nfree = np.abs(R[:, 4] - R[:, 3]) / np.abs(R[:, 3])
+    beta = np.abs(R[:, 6] - R[:, 5]) / R[:, 5]
+    eW, eY = _rel(Wb, Wa), _rel(Yb, Ya)
+    _say(f"  {r.nloop} iterations; max rel err llh {llh.max():.2e} -F {nfree.max():.2e} "
+         f"beta {beta.max():.2e} W {np.nanmax(R[:, 7]):.2e} Y {np.nanmax(R[:, 8]):.2e}; "
+         f"end llh {llh[-1]:.2e} beta {beta[-1]:.2e} W {eW:.2e} Y {eY:.2e}")
+    assert r.nloop > 1000
+    assert llh.max() < 2e-4 and nfree.max() < 2e-4 and beta.max() < 2e-4
+    assert np.nanmax(R[:, 7]) < 5e-2 and np.nanmax(R[:, 8]) < 3e-2
+    assert llh[-1] < 5e-6 and beta[-1] < 5e-6 and eW < 5e-5 and eY < 5e-5
