@@ -191,30 +191,12 @@ def device_peaks():
         return e.device_peaks()
 
 
-def c4_instances(n=64, seed=2024):
-    """The batched workload: independent instances of uneven N, D and K."""
-    rng = np.random.default_rng(seed)
-    shapes = [(2, 400, 25), (2, 1600, 25), (3, 8000, 125)]
-    out = []
-    for i in range(n):
-        N = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
-        D = int(rng.choice([12, 32, 64, 166]))
-        L, K, M = shapes[int(rng.integers(0, 3))]
-        s = int(rng.integers(1, 1 << 30))
-        T = (datagen.fingerprints(N, D, s).astype(np.float64) if D == 166
-             else datagen.gmm(N, D, s))
-        X, FI = datagen.latent_basis(L, K, M, 1.0)
-        W0, Y0, b0 = datagen.pca_init(T, FI, X)
-        out.append((T, FI, Y0, b0, dict(start_temp=float(rng.uniform(1.5, 4.0)))))
-    return out
-
-
 def batch_bench(args):
     """Instance-iterations/s of the batched workload: the annealing phase of
     dg_batch_anneal (every instance's data already resident), timed by the
     library, after one untimed batch of `warmup` iterations."""
     print("[bench] generating 64 instances", file=sys.stderr, flush=True)
-    probs = c4_instances()
+    probs = datagen.batch_instances()
     n = len(probs)
     prec = dg.MIXED if args.batch_precision == "mixed" else dg.FP64
     tuned = 0
@@ -243,6 +225,16 @@ def batch_bench(args):
             sys.exit(3)
     dt = ph[1] * 1e-3
     knd = sum(p[1].shape[0] * p[0].shape[0] * p[0].shape[1] for p in probs)
+    f_alg = 4.0 * knd * args.steps / dt / 1e12         # algorithmic TFLOP/s of the E-steps
+    pk_mfma = device_peaks()[0] if prec != dg.FP64 else None
+    if args.workers <= 0 and prec != dg.FP64:
+        mode = ("batched launches per step, " + ("replayed as hipGraphs" if args.workers == 0
+                                                  else "launched directly") +
+                "; E-passes as persistent launches over a device work queue")
+        eff_workers = 1
+    else:
+        eff_workers = min(4 if args.workers <= 0 else min(args.workers, 4), n)
+        mode = f"{eff_workers} host worker threads, one context and stream each"
     tbytes = sum(p[0].shape[0] * p[0].shape[1] * (1 if p[0].shape[1] == 166 else 4) for p in probs)
     it_s = n * args.steps / dt
     hbm = tbytes * args.steps / dt / 1e9
@@ -257,9 +249,14 @@ def batch_bench(args):
         "config": {"workload": "64 instances: N log-uniform [1e3, 2e5], D in {12,32,64,166}, "
                                "(L,K,M) in {(2,400,25),(2,1600,25),(3,8000,125)}, seed 2024",
                    "total_N": int(sum(p[0].shape[0] for p in probs)), "workers": args.workers,
-                   "tuned_shapes": tuned,
-                   "parallelism": "single GPU, one context and stream per worker"},
+                   "workers_effective": eff_workers, "tuned_shapes": tuned,
+                   "parallelism": "single GPU, " + mode},
         "kd_updates_per_s": knd * args.steps / dt,
+        "mfma": {"algorithmic_tflops": round(f_alg, 2), "peak_tflops": PEAK_FP16_TFLOPS,
+                 "frac": round(f_alg / PEAK_FP16_TFLOPS, 4),
+                 "measured_peak_tflops": None if pk_mfma is None else round(pk_mfma, 1),
+                 "frac_of_measured": None if pk_mfma is None else round(f_alg / pk_mfma, 4),
+                 "note": "F_alg = 4 K N D flop per instance-iteration"},
         "data_bytes_per_batch_step": tbytes,
         "hbm": {"achieved_GBps_data_once": round(hbm, 2), "peak_GBps": 8000.0,
                 "frac": round(hbm / 8000.0, 5),
@@ -284,7 +281,9 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="measure the 64-instance batched workload instead of the headline")
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--workers", type=int, default=0,
+                    help="with --batch: 0 = batched launches as hipGraphs, -1 = batched launches "
+                         "without graphs, 1..4 = host worker threads")
     ap.add_argument("--tune", action="store_true",
                     help="with --batch: tune the pass-2 chunk count of every distinct shape first "
                          "(dg_tune) and run the batch with that table")

@@ -400,9 +400,20 @@ int dg_mstep(dg_ctx *c, double *beta_out) {
     DG_HIP(c, hipMemcpyAsync(c->Gm, c->G, (size_t)c->K * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[4], c->stream));
-    if ((rc = dg_mstep_run(c))) return rc;
+    if ((rc = dg_mstep_run(c, 0.0))) return rc;
     if (c->timing) DG_HIP(c, hipEventRecord(c->ev[5], c->stream));
     if ((rc = sync_scalars(c))) return rc;
+    if (c->h_scal[kScalSolveOK] != 1.0 && c->M1 > 126) {
+        // a non-positive pivot: retry with a jitter of 1e-13 tr(A) / n, growing
+        // x10 per retry, at most 8 retries (the policy of the one-kernel
+        // factorisation that M + 1 <= 126 uses)
+        if ((rc = dg_mstep_trace(c)) || (rc = sync_scalars(c))) return rc;
+        double jit = 1e-13 * c->h_scal[kScalJitter] / c->M1;
+        for (int attempt = 1; attempt <= 8; ++attempt, jit *= 10.0) {
+            if ((rc = dg_mstep_run(c, jit)) || (rc = sync_scalars(c))) return rc;
+            if (c->h_scal[kScalSolveOK] == 1.0) break;
+        }
+    }
     if (c->timing) {
         float a = 0;
         (void)hipEventElapsedTime(&a, c->ev[4], c->ev[5]);
@@ -665,11 +676,14 @@ static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_i
         (void)hipGetLastError();
         return DG_EHIP;
     }
-    // at most kMaxBatchWorkers threads: one per hardware queue of the process
-    // (HIP's default of four); a run with eight workers and every instance
-    // resident stopped responding once, with no cause found in this code
-    if (workers <= 0 || workers > kMaxBatchWorkers) workers = kMaxBatchWorkers;
-    workers = std::min(workers, n);
+    // workers 0 / -1: the batched launch sequence (with / without hipGraph
+    // capture) on this thread, for the mixed precisions; otherwise host
+    // workers, at most kMaxBatchWorkers: one per hardware queue of the
+    // process (HIP's default of four)
+    const bool batched = workers <= 0 && cfg->precision != DG_FP64;
+    const bool graphs = workers == 0;
+    if (batched || workers <= 0 || workers > kMaxBatchWorkers) workers = kMaxBatchWorkers;
+    workers = batched ? 1 : std::min(workers, n);
     // largest work first, so the tail of the queue holds the short items
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
@@ -701,12 +715,31 @@ static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_i
                                 it.Y0, it.ldY0, it.beta0);
         status[i] = rc;
     });
-    // phase 2: the annealing runs, concurrently on the workers' host threads
+    // phase 2: the annealing runs, concurrently on the workers' host threads,
+    // or all instances' steps as batched launches
     const auto t1 = clk::now();
-    run_pool([&](int i) {
-        if (status[i] != DG_OK) return;
-        status[i] = dg_anneal(ctx[i], &items[i].params, &res[i], nullptr, nullptr);
-    });
+    int brc = DG_OK;
+    if (batched) {
+        std::vector<int> ok;
+        for (int i = 0; i < n; ++i)
+            if (status[i] == DG_OK) ok.push_back(i);
+        std::vector<dg_ctx *> cs;
+        std::vector<dg_anneal_params> ps;
+        std::vector<dg_anneal_result> rs(ok.size());
+        std::vector<int> st(ok.size(), DG_OK);
+        for (int i : ok) { cs.push_back(ctx[i]); ps.push_back(items[i].params); }
+        if (!ok.empty() && (brc = select_device(ctx[ok[0]])) == DG_OK)
+            brc = dg_batch_run_mixed(cs, ps.data(), rs.data(), st.data(), graphs);
+        for (size_t j = 0; j < ok.size(); ++j) {
+            res[ok[j]] = rs[j];
+            status[ok[j]] = brc ? brc : st[j];
+        }
+    } else {
+        run_pool([&](int i) {
+            if (status[i] != DG_OK) return;
+            status[i] = dg_anneal(ctx[i], &items[i].params, &res[i], nullptr, nullptr);
+        });
+    }
     const auto t2 = clk::now();
     for (int i = 0; i < n; ++i) {
         const dg_batch_item &it = items[i];
@@ -718,7 +751,7 @@ static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_i
         phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
         phase_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
     }
-    return DG_OK;
+    return brc == DG_EHIP ? DG_EHIP : DG_OK;
 }
 
 int dg_map(dg_ctx *c, const double *X, int L, long ldX, double *M, long ldM) {

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/dagtm_gpu.h"
 
@@ -119,6 +120,44 @@ struct BInst {
     int item_e1, item_e2;
 };
 
+// largest shapes over the instances of a batched launch (grid extents)
+struct BShape {
+    long Np = 0, Kp = 0;
+    int K = 0, D = 0, M1 = 0;
+};
+
+// instances of one E-pass template class in a batched launch: their table
+// indices are list[first .. first + n), with items_e1 / items_e2 work items
+// in all; index numbers the class's two work-queue counters
+struct BClass {
+    int KS, TT, D, two, rlo;
+    int first, n, items_e1, items_e2, index;
+};
+
+// batched E-step pieces (dg_mixed.hip)
+int dg_batch_fill_mixed(const dg_ctx *c, BInst *I);
+int dg_batch_items(const BInst &I, int *e1, int *e2);
+// make_ak, pass 1 per class, the log sums, pass 2 per class, the chunk
+// reductions and the zero-column fix of the n listed instances; counters
+// holds 4 ncls ints of device memory
+int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
+                   int ncls, int *counters, const BShape &mx, int cus);
+// the FP16 split of the new centres (dg_mixed_prepare_y) of the listed instances
+int dg_batch_prepare_y(hipStream_t st, const BInst *tab, const int *list, int n,
+                       const BShape &mx);
+// out[i][kScalCount] = the scalars of table entry i, for i < n
+int dg_batch_gather_scal(hipStream_t st, const BInst *tab, int n, double *out);
+
+// M-step of the listed instances (dg_mstep.hip): list holds n table indices,
+// the nlds instances with M + 1 <= 126 (LDS Cholesky) first
+int dg_batch_mstep(hipStream_t st, const BInst *tab, const int *list, int n, int nlds,
+                   const BShape &mx);
+
+// annealing of n uploaded mixed-precision contexts as batched launches per
+// step (dg_batch.cpp); graphs: capture each phase once as a hipGraph
+int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *params,
+                       dg_anneal_result *res, int *status, bool graphs);
+
 int dg_fail(dg_ctx *ctx, int code, const std::string &msg);
 int dg_hip_check(dg_ctx *ctx, hipError_t e, const char *what);
 
@@ -151,7 +190,11 @@ int dg_estep_mixed(dg_ctx *ctx, int isDA, double temp);
 void dg_mixed_free(dg_ctx *ctx);
 
 // ---- M-step (dg_mstep.hip) ----
-int dg_mstep_run(dg_ctx *ctx);
+// The M-step kernels; for M + 1 > 126 the factorisation uses A0 + (ridge +
+// jitter) I (M + 1 <= 126 retries with a jitter inside its one kernel).
+int dg_mstep_run(dg_ctx *ctx, double jitter);
+// tr(A0) into scal[kScalJitter] (for the retries of M + 1 > 126)
+int dg_mstep_trace(dg_ctx *ctx);
 // Y = FI W and |y_k|^2 from the context's FI and W.
 int dg_centres_from_w(dg_ctx *ctx);
 int dg_rownorms(dg_ctx *ctx, const double *X, long rows, int ld, int D, double *out);

@@ -93,7 +93,7 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 // workgroup, kb_e1_mfma takes tiles of many instances from a work queue.
 template <int KS, int TT, bool TWO>
 __device__ __forceinline__ void e1_body(
-    const long bx, const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N,
+    const long bx, const int tid, const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N,
     long Np, const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
@@ -121,7 +121,7 @@ __device__ __forceinline__ void e1_body(
     auto sp = [&](int s) -> _Float16 * { return s ? smem1 : smem0; };
     __shared__ double wsum[3][4];
 
-    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
+    const int w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
     const int wu = __builtin_amdgcn_readfirstlane(w);
     const long n = bx * PT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
@@ -389,13 +389,14 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
     double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
-    e1_body<KS, TT, TWO>(blockIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal, beta, it, isDA, logK,
+    e1_body<KS, TT, TWO>(blockIdx.x, threadIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal, beta, it, isDA, logK,
                          pP, blkThr, pm, logg, logh, partLog);
 }
 
 // scal[0..2] = sums over the pass-1 workgroup partials (fixed tree order).
-__global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__ part, long nb,
-                                                      double *__restrict__ scal) {
+__device__ __forceinline__ void k_sum_partlog_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ part,
+    long nb, double *__restrict__ scal) {
     __shared__ double s[3][1024];
     double a = 0.0, b = 0.0, c = 0.0;
     for (long i = threadIdx.x; i < nb; i += 1024) {
@@ -418,13 +419,24 @@ __global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__
     }
 }
 
+__global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__ part, long nb,
+                                                      double *__restrict__ scal) {
+    k_sum_partlog_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, part, nb, scal);
+}
+
 // Per-centre exponent term of pass 1: aK32[k] = -beta log2(e) |y_k|^2 / 2
 // (the same FP32 expression pass 2 uses), -inf for padding centres.
-__global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float beta,
-                          float *__restrict__ aK32) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_make_ak_body(
+    const int bx, const int by, const int bz, const int gx, const float *__restrict__ yy32,
+    int K, int Kp, float beta, float *__restrict__ aK32) {
+    const int k = bx * blockDim.x + threadIdx.x;
     const float cb = -0.5f * beta * kLog2e;
     if (k < Kp) aK32[k] = (k < K) ? cb * yy32[k] : -INFINITY;
+}
+
+__global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float beta,
+                          float *__restrict__ aK32) {
+    k_make_ak_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, yy32, K, Kp, beta, aK32);
 }
 
 // ---------------------------------------------------------------- pass 2
@@ -450,7 +462,7 @@ __global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float b
 // work queue over many instances.
 template <int KS, int TT, bool GM, bool RLO>
 __device__ __forceinline__ void e2_body(
-    const int L, const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
+    const int L, const int tid, const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
@@ -478,7 +490,7 @@ __device__ __forceinline__ void e2_body(
     const int xs = L & 7, sl = L >> 3;
     const int kg = sl % KG;
     const long cidx = (long)(sl / KG) * 8 + xs;
-    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
+    const int w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
     const int wu = __builtin_amdgcn_readfirstlane(w);
     const int k = kg * CT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
@@ -835,7 +847,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
     float *__restrict__ partRT, float *__restrict__ partG) {
-    e2_body<KS, TT, GM, RLO>(blockIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D, mscal, beta, it, pP,
+    e2_body<KS, TT, GM, RLO>(blockIdx.x, threadIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D, mscal, beta, it, pP,
                              blkThr, chunk, KG, partRT, partG);
 }
 
@@ -848,13 +860,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // Workgroups of the first dim tile also sum G.
 constexpr int RK = 64, RD = 16, RU = 8;
 
-__global__ __launch_bounds__(RK * RD) void k_reduce32(const float *__restrict__ partRT,
-                                                      const float *__restrict__ partG, int NC,
-                                                      int K, int D, double *__restrict__ RT,
-                                                      double *__restrict__ G) {
+__device__ __forceinline__ void k_reduce32_body(
+    const int bx, const int by, const int bz, const int gx, const float *__restrict__ partRT,
+    const float *__restrict__ partG, int NC, int K, int D, double *__restrict__ RT,
+    double *__restrict__ G) {
     __shared__ double tile[RK][RD + 1];
     const int t = threadIdx.x, kk = t & (RK - 1), dd = t / RK;
-    const int k0 = blockIdx.x * RK, d0 = blockIdx.y * RD;
+    const int k0 = bx * RK, d0 = by * RD;
     const int k = k0 + kk, d = d0 + dd;
     const long KD = (long)K * D;
     double s = 0.0;
@@ -876,24 +888,30 @@ __global__ __launch_bounds__(RK * RD) void k_reduce32(const float *__restrict__ 
         const int r = t / RD, e = t % RD;
         if (k0 + r < K && d0 + e < D) RT[(long)(k0 + r) * D + d0 + e] = tile[r][e];
     }
-    if (blockIdx.y == 0 && t < RK && k < K) {
+    if (by == 0 && t < RK && k < K) {
         double g = 0.0;
         for (int c = 0; c < NC; ++c) g += (double)partG[(long)c * K + k];
         G[k] = g;
     }
 }
 
+__global__ __launch_bounds__(RK * RD) void k_reduce32(const float *__restrict__ partRT,
+                                                      const float *__restrict__ partG, int NC,
+                                                      int K, int D, double *__restrict__ RT,
+                                                      double *__restrict__ G) {
+    k_reduce32_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, partRT, partG, NC, K, D, RT, G);
+}
+
 // Zero-column fall-back R = 1/K for the points pass 2 skipped: RT[k] +=
 // (1/K) sum_{n in Z} t_n, G[k] += |Z| / K.  Exits at once when |Z| = 0.
-__global__ __launch_bounds__(256) void k_zero_fix(const double *__restrict__ scal,
-                                                  const double *__restrict__ T64, int ldt,
-                                                  const double *__restrict__ pm, long N, int K,
-                                                  int D, double *__restrict__ RT,
-                                                  double *__restrict__ G) {
+__device__ __forceinline__ void k_zero_fix_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ scal,
+    const double *__restrict__ T64, int ldt, const double *__restrict__ pm, long N, int K,
+    int D, double *__restrict__ RT, double *__restrict__ G) {
     const double nz = scal[kScalNZero];
     if (nz == 0.0) return;
     __shared__ double red[256];
-    const int d = blockIdx.x;
+    const int d = bx;
     double s = 0.0;
     for (long n = threadIdx.x; n < N; n += 256)
         if (pm[n] == -INFINITY) s += T64[n * ldt + d];
@@ -910,15 +928,25 @@ __global__ __launch_bounds__(256) void k_zero_fix(const double *__restrict__ sca
     }
 }
 
+__global__ __launch_bounds__(256) void k_zero_fix(const double *__restrict__ scal,
+                                                  const double *__restrict__ T64, int ldt,
+                                                  const double *__restrict__ pm, long N, int K,
+                                                  int D, double *__restrict__ RT,
+                                                  double *__restrict__ G) {
+    k_zero_fix_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, scal, T64, ldt, pm, N, K, D, RT,
+                    G);
+}
+
 // Workgroup partials of max |x| and of the count of values not exact in
 // FP16 over rows x D (stride ld): part[2 b], part[2 b + 1].
 constexpr int kAmaxWG = 256;
-__global__ __launch_bounds__(256) void k_absmax_part(const double *__restrict__ X, long rows,
-                                                     int ld, int D, double *__restrict__ part) {
+__device__ __forceinline__ void k_absmax_part_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ X,
+    long rows, int ld, int D, double *__restrict__ part) {
     __shared__ double smax[256], scnt[256];
     double mx = 0.0, cnt = 0.0;
     const long total = rows * D;
-    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    for (long e = (long)bx * 256 + threadIdx.x; e < total; e += (long)gx * 256) {
         const double v = X[(e / D) * ld + (e % D)];
         mx = fmax(mx, fabs(v));
         const _Float16 hv = (_Float16)(float)v;
@@ -933,11 +961,17 @@ __global__ __launch_bounds__(256) void k_absmax_part(const double *__restrict__ 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = smax[0]; part[2 * blockIdx.x + 1] = scnt[0]; }
+    if (threadIdx.x == 0) { part[2 * bx] = smax[0]; part[2 * bx + 1] = scnt[0]; }
 }
 
-__global__ __launch_bounds__(256) void k_absmax_final(const double *__restrict__ part, int n,
-                                                      double *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_absmax_part(const double *__restrict__ X, long rows,
+                                                     int ld, int D, double *__restrict__ part) {
+    k_absmax_part_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, X, rows, ld, D, part);
+}
+
+__device__ __forceinline__ void k_absmax_final_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ part,
+    int n, double *__restrict__ out) {
     __shared__ double smax[256], scnt[256];
     double mx = 0.0, cnt = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) { mx = fmax(mx, part[2 * i]); cnt += part[2 * i + 1]; }
@@ -951,6 +985,11 @@ __global__ __launch_bounds__(256) void k_absmax_final(const double *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) { out[0] = smax[0]; out[1] = scnt[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_absmax_final(const double *__restrict__ part, int n,
+                                                      double *__restrict__ out) {
+    k_absmax_final_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, part, n, out);
 }
 
 // Power-of-two scale putting max |x| at or below 2^14.
@@ -1016,16 +1055,14 @@ __global__ void k_transpose16(const _Float16 *__restrict__ T16, long N, long Np,
 // Split Y (K x D, stride ldt, FP64) into two FP16 terms with a power-of-two
 // scale chosen from scal_in[0] = max |Y|: Y16[t][k][d], rows k >= K zero;
 // yy32[k] = |y'|^2 of the split value; mscal[0] = the scale.
-__global__ __launch_bounds__(256) void k_make_y16(const double *__restrict__ Y64, int ldt, int K,
-                                                  int Kp, int D, int ld16,
-                                                  const double *__restrict__ amax,
-                                                  _Float16 *__restrict__ Y16,
-                                                  float *__restrict__ yy32,
-                                                  double *__restrict__ mscal) {
+__device__ __forceinline__ void k_make_y16_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ Y64,
+    int ldt, int K, int Kp, int D, int ld16, const double *__restrict__ amax,
+    _Float16 *__restrict__ Y16, float *__restrict__ yy32, double *__restrict__ mscal) {
     // one wave per centre row; lanes step over d, the norm is a wave sum
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int k = bx * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
     const double scale = pow2_scale(amax[0]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) mscal[0] = scale;
+    if (bx == 0 && threadIdx.x == 0) mscal[0] = scale;
     if (k >= Kp) return;
     double ss = 0.0;
     for (int d = l; d < ld16; d += 64) {
@@ -1040,6 +1077,16 @@ __global__ __launch_bounds__(256) void k_make_y16(const double *__restrict__ Y64
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
     if (l == 0) yy32[k] = (float)ss;
+}
+
+__global__ __launch_bounds__(256) void k_make_y16(const double *__restrict__ Y64, int ldt, int K,
+                                                  int Kp, int D, int ld16,
+                                                  const double *__restrict__ amax,
+                                                  _Float16 *__restrict__ Y16,
+                                                  float *__restrict__ yy32,
+                                                  double *__restrict__ mscal) {
+    k_make_y16_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, Y64, ldt, K, Kp, D, ld16, amax,
+                    Y16, yy32, mscal);
 }
 
 // Supported numbers of 16-wide d steps (D is padded up to the next one).
@@ -1139,6 +1186,233 @@ int dalloc(dg_ctx *c, T **p, size_t count) {
         (void)hipGetLastError();
         return dg_fail(c, DG_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     }
+    return DG_OK;
+}
+
+// ---------------------------------------------------------------- batched
+// The E-step of many instances per step.  Both E-passes run as persistent
+// grids (as many workgroups as fit on the device) over a work queue: a
+// workgroup takes the next item from a device counter, finds its instance in
+// the launch's list (items are numbered instance after instance, each
+// instance's first item in its table entry), runs the single-instance
+// workgroup body on it, and stops when the queue is empty.  Every item
+// writes its own outputs, so results do not depend on the order items run
+// in: each instance gets exactly its single-instance results.  The light
+// kernels take one grid dimension as the list slot and return at once in
+// workgroups past the instance's own grid.
+
+// the list entry whose first item is the largest one <= item (list ordered
+// by first item); item is wave-uniform, so this runs on scalar loads
+template <bool E2>
+__device__ __forceinline__ const BInst &find_inst(const BInst *__restrict__ tab,
+                                                  const int *__restrict__ list, int nlist,
+                                                  int item) {
+    int lo = 0, hi = nlist - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int first = E2 ? tab[list[mid]].item_e2 : tab[list[mid]].item_e1;
+        if (first <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return tab[list[lo]];
+}
+
+// next work item for the whole workgroup (uniform), or >= total when none
+__device__ __forceinline__ int next_item(int *__restrict__ counter, int *s_item) {
+    __syncthreads();                      // every wave is done with the previous item
+    if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*s_item);
+}
+
+// the thread index, redefined (an empty asm) in every work-loop iteration:
+// the bodies' lane-derived values are then recomputed per item instead of
+// being hoisted out of the loop and held in registers across it
+__device__ __forceinline__ int opaque_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+template <int KS, int TT, bool TWO>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void kb_e1_mfma(
+    const BInst *__restrict__ tab, const int *__restrict__ list, int nlist, int total,
+    int *__restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        const int item = next_item(counter, &s_item);
+        if (item >= total) break;
+        const BInst &I = find_inst<false>(tab, list, nlist, item);
+        if (!I.do_e) continue;
+        e1_body<KS, TT, TWO>(item - I.item_e1, opaque_tid(), I.T16, I.tt32, I.N, I.Np, I.Y16, I.aK32, I.Kp,
+                             I.mscal, I.beta_f, I.it, I.isDA, I.logK, I.pP32, I.blkThr, I.pm,
+                             I.logg, I.logh, I.partLog);
+    }
+}
+
+template <int KS, int TT, bool GM, bool RLO>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) void kb_e2_mfma(
+    const BInst *__restrict__ tab, const int *__restrict__ list, int nlist, int total,
+    int *__restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        const int item = next_item(counter, &s_item);
+        if (item >= total) break;
+        const BInst &I = find_inst<true>(tab, list, nlist, item);
+        if (!I.do_e) continue;
+        e2_body<KS, TT, GM, RLO>(item - I.item_e2, opaque_tid(), I.T16, I.T16t, I.Np, I.Y16, I.yy32, I.K, I.Kp,
+                                 I.D, I.mscal, I.beta_f, I.it, I.pP32, I.blkThr, I.CL,
+                                 I.Kp / CT, I.partRT32, I.partG32);
+    }
+}
+
+__global__ void kb_zero_ints(int *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void kb_make_ak(const BInst *__restrict__ tab,
+                                                  const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_e || (int)blockIdx.x * 256 >= I.Kp) return;
+    k_make_ak_body(blockIdx.x, 0, 0, 0, I.yy32, I.K, I.Kp, I.beta_f, I.aK32);
+}
+
+__global__ __launch_bounds__(1024) void kb_sum_partlog(const BInst *__restrict__ tab,
+                                                       const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.x]];
+    if (!I.do_e) return;
+    k_sum_partlog_body(0, 0, 0, 1, I.partLog, I.Np / PT, I.scal);
+}
+
+__global__ __launch_bounds__(RK * RD) void kb_reduce32(const BInst *__restrict__ tab,
+                                                       const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.z]];
+    if (!I.do_e || (int)blockIdx.x * RK >= I.K || (int)blockIdx.y * RD >= I.D) return;
+    k_reduce32_body(blockIdx.x, blockIdx.y, 0, 0, I.partRT32, I.partG32, I.NC, I.K, I.D, I.RT,
+                    I.G);
+}
+
+__global__ __launch_bounds__(256) void kb_zero_fix(const BInst *__restrict__ tab,
+                                                   const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_e || (int)blockIdx.x >= I.D) return;
+    k_zero_fix_body(blockIdx.x, 0, 0, 0, I.scal, I.T64, I.ldt, I.pm, I.N, I.K, I.D, I.RT, I.G);
+}
+
+__device__ __forceinline__ int amax_wgs(const BInst &I) {
+    const long w = ((long)I.K * I.D + 4095) / 4096;
+    return (int)(w < kAmaxWG ? w : kAmaxWG);
+}
+
+__global__ __launch_bounds__(256) void kb_absmax_part(const BInst *__restrict__ tab,
+                                                      const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    const int nwg = amax_wgs(I);
+    if (!I.do_m || (int)blockIdx.x >= nwg) return;
+    k_absmax_part_body(blockIdx.x, 0, 0, nwg, I.Y64, (long)I.K, I.ldt, I.D, I.amaxPart);
+}
+
+__global__ __launch_bounds__(256) void kb_absmax_final(const BInst *__restrict__ tab,
+                                                       const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.x]];
+    if (!I.do_m) return;
+    k_absmax_final_body(0, 0, 0, 1, I.amaxPart, amax_wgs(I), I.scal + 6);
+}
+
+__global__ __launch_bounds__(256) void kb_make_y16(const BInst *__restrict__ tab,
+                                                   const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_m || (int)blockIdx.x * 4 >= I.Kp) return;
+    k_make_y16_body(blockIdx.x, 0, 0, 0, I.Y64, I.ldt, I.K, I.Kp, I.D, I.ldh, I.scal + 6, I.Y16,
+                    I.yy32, I.mscal);
+}
+
+// out[i][0 .. kScalCount) = the scalars of table entry i
+__global__ void kb_gather_scal(const BInst *__restrict__ tab, int n, double *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * kScalCount) out[e] = tab[e / kScalCount].scal[e % kScalCount];
+}
+
+// persistent grid size of a kernel: the workgroups that fit on the device
+// (the occupancy query runs once per kernel, before any graph capture uses it)
+template <typename F>
+int resident_wgs(F kernel, int cus) {
+    static const int per = [&] {
+        int p = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, kernel, NTH, 0) != hipSuccess ||
+            p < 1)
+            p = 1;
+        return p;
+    }();
+    return per * cus;
+}
+
+template <int KS, int TT>
+void launch_e1_batch(hipStream_t st, const BInst *tab, const int *list, int n, int total,
+                     int *counter, bool two, int cus) {
+    if (two) {
+        const int g = std::min(total, resident_wgs(kb_e1_mfma<KS, TT, true>, cus));
+        hipLaunchKernelGGL((kb_e1_mfma<KS, TT, true>), dim3((unsigned)g), dim3(NTH), 0, st, tab,
+                           list, n, total, counter);
+    } else {
+        const int g = std::min(total, resident_wgs(kb_e1_mfma<KS, TT, false>, cus));
+        hipLaunchKernelGGL((kb_e1_mfma<KS, TT, false>), dim3((unsigned)g), dim3(NTH), 0, st, tab,
+                           list, n, total, counter);
+    }
+}
+
+template <int KS, int TT, bool GM>
+void launch_e2_batch_gm(hipStream_t st, const BInst *tab, const int *list, int n, int total,
+                        int *counter, bool rlo, int cus) {
+    if (rlo) {
+        const int g = std::min(total, resident_wgs(kb_e2_mfma<KS, TT, GM, true>, cus));
+        hipLaunchKernelGGL((kb_e2_mfma<KS, TT, GM, true>), dim3((unsigned)g), dim3(NTH), 0, st,
+                           tab, list, n, total, counter);
+    } else {
+        const int g = std::min(total, resident_wgs(kb_e2_mfma<KS, TT, GM, false>, cus));
+        hipLaunchKernelGGL((kb_e2_mfma<KS, TT, GM, false>), dim3((unsigned)g), dim3(NTH), 0, st,
+                           tab, list, n, total, counter);
+    }
+}
+
+template <int KS, int TT>
+void launch_e2_batch(hipStream_t st, const BInst *tab, const int *list, int n, int total,
+                     int *counter, bool rlo, int D, int cus) {
+    constexpr int DR = 32 * ((16 * KS + 31) / 32);
+    if (DR > D) launch_e2_batch_gm<KS, TT, true>(st, tab, list, n, total, counter, rlo, cus);
+    else launch_e2_batch_gm<KS, TT, false>(st, tab, list, n, total, counter, rlo, cus);
+}
+
+template <int TT>
+int launch_class(const BClass &k, hipStream_t st, const BInst *tab, const int *list,
+                 int *counters, bool pass2, int cus) {
+    const int *l = list + k.first;
+    int *ctr = counters + 2 * k.index + (pass2 ? 1 : 0);
+#define DG_BATCH_KS(V)                                                                       \
+    case V:                                                                                  \
+        if (pass2) launch_e2_batch<V, TT>(st, tab, l, k.n, k.items_e2, ctr, k.rlo, k.D, cus); \
+        else launch_e1_batch<V, TT>(st, tab, l, k.n, k.items_e1, ctr, k.two, cus);           \
+        break;
+    switch (k.KS) {
+        DG_BATCH_KS(1)
+        DG_BATCH_KS(2)
+        DG_BATCH_KS(3)
+        DG_BATCH_KS(4)
+        DG_BATCH_KS(6)
+        DG_BATCH_KS(8)
+        default:
+            if constexpr (TT == 1) {
+                switch (k.KS) {
+                    DG_BATCH_KS(11)
+                    DG_BATCH_KS(12)
+                    default: return DG_EINVAL;
+                }
+            } else {
+                return DG_EINVAL;
+            }
+    }
+#undef DG_BATCH_KS
     return DG_OK;
 }
 
@@ -1311,4 +1585,67 @@ int dg_estep_mixed(dg_ctx *c, int isDA, double temp) {
                        c->ldt, c->pm, c->N, c->K, c->D, c->RT, c->G);
     DG_HIP(c, hipGetLastError());
     return DG_OK;
+}
+
+// ---- batched E-step ---------------------------------------------------------
+int dg_batch_fill_mixed(const dg_ctx *c, BInst *I) {
+    const MixedBufs *b = reinterpret_cast<const MixedBufs *>(c->mix);
+    if (!b || !c->mixed_ready) return DG_ESTATE;
+    I->Np = b->Np; I->CL = b->CL; I->Kp = b->Kp; I->ldh = c->ldh; I->NC = b->NC; I->KS = b->KS;
+    I->TT = b->TT;
+    I->T16 = (const _Float16 *)c->T16; I->T16t = b->T16t; I->Y16 = b->Y16;
+    I->tt32 = c->tt32; I->yy32 = c->yy32; I->aK32 = b->aK32; I->pP32 = c->pP32;
+    I->blkThr = b->blkThr; I->partRT32 = c->partRT32; I->partG32 = c->partG32;
+    I->mscal = b->mscal; I->partLog = b->partLog; I->amaxPart = b->amaxPart;
+    return DG_OK;
+}
+
+int dg_batch_items(const BInst &I, int *e1, int *e2) {
+    *e1 = (int)(I.Np / PT);
+    *e2 = (I.Kp / CT) * I.NC;
+    return DG_OK;
+}
+
+int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
+                   int ncls, int *counters, const BShape &mx, int cus) {
+    if (n <= 0) return DG_OK;
+    const unsigned un = (unsigned)n;
+    hipLaunchKernelGGL(kb_zero_ints, dim3((unsigned)((4 * ncls + 255) / 256)), dim3(256), 0, st,
+                       counters, 4 * ncls);
+    hipLaunchKernelGGL(kb_make_ak, dim3((unsigned)((mx.Kp + 255) / 256), un), dim3(256), 0, st,
+                       tab, list);
+    int rc;
+    for (int i = 0; i < ncls; ++i) {
+        rc = (cls[i].TT == 1) ? launch_class<1>(cls[i], st, tab, list, counters, false, cus)
+                              : launch_class<2>(cls[i], st, tab, list, counters, false, cus);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(kb_sum_partlog, dim3(un), dim3(1024), 0, st, tab, list);
+    for (int i = 0; i < ncls; ++i) {
+        rc = (cls[i].TT == 1) ? launch_class<1>(cls[i], st, tab, list, counters, true, cus)
+                              : launch_class<2>(cls[i], st, tab, list, counters, true, cus);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(kb_reduce32, dim3((unsigned)((mx.K + RK - 1) / RK), (unsigned)((mx.D + RD - 1) / RD), un),
+                       dim3(RK * RD), 0, st, tab, list);
+    hipLaunchKernelGGL(kb_zero_fix, dim3((unsigned)mx.D, un), dim3(256), 0, st, tab, list);
+    return hipGetLastError() == hipSuccess ? DG_OK : DG_EHIP;
+}
+
+int dg_batch_prepare_y(hipStream_t st, const BInst *tab, const int *list, int n,
+                       const BShape &mx) {
+    if (n <= 0) return DG_OK;
+    const unsigned un = (unsigned)n;
+    hipLaunchKernelGGL(kb_absmax_part, dim3((unsigned)kAmaxWG, un), dim3(256), 0, st, tab, list);
+    hipLaunchKernelGGL(kb_absmax_final, dim3(un), dim3(256), 0, st, tab, list);
+    hipLaunchKernelGGL(kb_make_y16, dim3((unsigned)((mx.Kp + 3) / 4), un), dim3(256), 0, st, tab,
+                       list);
+    return hipGetLastError() == hipSuccess ? DG_OK : DG_EHIP;
+}
+
+int dg_batch_gather_scal(hipStream_t st, const BInst *tab, int n, double *out) {
+    if (n <= 0) return DG_OK;
+    hipLaunchKernelGGL(kb_gather_scal, dim3((unsigned)((n * kScalCount + 255) / 256)), dim3(256), 0,
+                       st, tab, n, out);
+    return hipGetLastError() == hipSuccess ? DG_OK : DG_EHIP;
 }

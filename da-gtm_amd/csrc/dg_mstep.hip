@@ -16,9 +16,10 @@
 namespace {
 
 // out[r] = sum_d X[r][d]^2, sequential in d (the ddot loops of comm.c:604-618)
-__global__ void k_rownorm(const double *__restrict__ X, long rows, int ld, int D,
-                          double *__restrict__ out) {
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_rownorm_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ X,
+    long rows, int ld, int D, double *__restrict__ out) {
+    const long r = (long)bx * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     double s = 0.0;
     for (int d = 0; d < D; ++d) {
@@ -26,6 +27,11 @@ __global__ void k_rownorm(const double *__restrict__ X, long rows, int ld, int D
         s = fma(v, v, s);
     }
     out[r] = s;
+}
+
+__global__ void k_rownorm(const double *__restrict__ X, long rows, int ld, int D,
+                          double *__restrict__ out) {
+    k_rownorm_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, X, rows, ld, D, out);
 }
 
 // Deterministic block-tree sum of one or two vectors into scal[slot], scal[slot+1].
@@ -83,17 +89,17 @@ __device__ void block_trace(const double *__restrict__ A0, int n, double *out) {
     __syncthreads();
 }
 
-// Cholesky solve of (A0 + (ridge + jitter) I) W = B in one workgroup, the
-// factor built in place in A (right-looking; each entry receives its
-// updates in the same column order as a left-looking loop).  On a
+// Cholesky solve of (A0 + (ridge + jitter) I) W = B in one workgroup (the
+// batched M-step's path for M + 1 > 126; a single instance uses the blocked
+// kernels below), the factor built in place in A (right-looking; each entry
+// receives its updates in the same column order as a left-looking loop).  On a
 // non-positive pivot the factorisation restarts with jitter 1e-13 tr/n,
 // growing x10 per retry, at most 8 retries.  scal[kScalSolveOK] = 1 on
 // success, scal[kScalJitter] = the jitter used.
-__global__ __launch_bounds__(1024) void k_chol_solve(const double *__restrict__ A0,
-                                                     double *__restrict__ A, int n,
-                                                     double ridge, const double *__restrict__ B,
-                                                     int nrhs, double *__restrict__ Wout,
-                                                     double *__restrict__ scal) {
+__device__ __forceinline__ void k_chol_solve_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ A0,
+    double *__restrict__ A, int n, double ridge, const double *__restrict__ B, int nrhs,
+    double *__restrict__ Wout, double *__restrict__ scal) {
     __shared__ int fail;
     __shared__ double tr;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -160,6 +166,252 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double *__restrict__ 
     }
 }
 
+// ---- blocked Cholesky and solves for M + 1 > 126 ---------------------------
+// (A0 + (ridge + jitter) I) W = B factored and solved by 64 x 64 blocks over
+// many workgroups, one launch per step: for each diagonal block k the block
+// is factored in LDS (k_bc_potrf), the panel below it solved against it
+// (k_bc_trsm) and the trailing lower triangle updated (k_bc_syrk); then the
+// forward and backward substitutions run block by block, a triangular solve
+// of the block's rows (k_bc_fwd / k_bc_bwd) and a product that moves its
+// contribution into the remaining rows (k_bc_fwd_update / k_bc_bwd_update).
+// A non-positive pivot clears scal[kScalSolveOK]; every later kernel of the
+// sequence then returns at once, and the host retries with a jitter.
+constexpr int BC = 64;
+
+__global__ __launch_bounds__(256) void k_bc_init(const double *__restrict__ A0,
+                                                 double *__restrict__ A, int n, double ridge,
+                                                 double jit, const double *__restrict__ B,
+                                                 double *__restrict__ X, int nrhs,
+                                                 double *__restrict__ scal) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nn = (long)n * n;
+    if (e < nn) A[e] = A0[e] + ((e / n == e % n) ? ridge + jit : 0.0);
+    else if (e < nn + (long)n * nrhs) X[e - nn] = B[e - nn];
+    if (e == 0) {
+        scal[kScalSolveOK] = 1.0;
+        scal[kScalJitter] = jit;
+    }
+}
+
+// factor the kb x kb diagonal block at k0 in LDS (right-looking, column by column)
+__global__ __launch_bounds__(256) void k_bc_potrf(double *__restrict__ A, int n, int k0,
+                                                  double *__restrict__ scal) {
+    __shared__ double L[BC][BC + 1];
+    __shared__ int bad;
+    if (scal[kScalSolveOK] != 1.0) return;
+    const int tid = threadIdx.x;
+    const int kb = min(BC, n - k0);
+    for (int e = tid; e < kb * kb; e += 256) L[e / kb][e % kb] = A[(long)(k0 + e / kb) * n + k0 + e % kb];
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (tid == 0) {
+            const double p = L[j][j];
+            if (!(p > 0.0)) bad = 1;
+            else L[j][j] = sqrt(p);
+        }
+        __syncthreads();
+        if (bad) break;
+        const double ljj = L[j][j];
+        for (int i = j + 1 + tid; i < kb; i += 256) L[i][j] /= ljj;
+        __syncthreads();
+        const int m = kb - j - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int r = j + 1 + e / m, c = j + 1 + e % m;
+            if (c <= r) L[r][c] = fma(-L[r][j], L[c][j], L[r][c]);
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) scal[kScalSolveOK] = 0.0;
+        return;
+    }
+    for (int e = tid; e < kb * kb; e += 256) {
+        const int r = e / kb, c = e % kb;
+        if (c <= r) A[(long)(k0 + r) * n + k0 + c] = L[r][c];
+    }
+}
+
+// rows i >= k0 + kb of the panel: L21 = A21 L11^-T, one row per thread
+__global__ __launch_bounds__(256) void k_bc_trsm(double *__restrict__ A, int n, int k0,
+                                                 const double *__restrict__ scal) {
+    __shared__ double L[BC][BC + 1];
+    if (scal[kScalSolveOK] != 1.0) return;
+    const int tid = threadIdx.x;
+    const int kb = min(BC, n - k0), k1 = k0 + kb;
+    for (int e = tid; e < kb * kb; e += 256) L[e / kb][e % kb] = A[(long)(k0 + e / kb) * n + k0 + e % kb];
+    __syncthreads();
+    const long i = (long)k1 + (long)blockIdx.x * 256 + tid;
+    if (i >= n) return;
+    double x[BC];
+#pragma unroll
+    for (int j = 0; j < BC; ++j) x[j] = (j < kb) ? A[i * n + k0 + j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+        if (j < kb) {
+            double s = x[j];
+#pragma unroll
+            for (int p = 0; p < j; ++p) s = fma(-x[p], L[j][p], s);
+            x[j] = s / L[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BC; ++j)
+        if (j < kb) A[i * n + k0 + j] = x[j];
+}
+
+// trailing update of the lower triangle: A[i][j] -= sum_p L[i][p] L[j][p]
+// over the panel columns p, for k1 <= j <= i < n; 64 x 64 tiles (ti >= tj)
+__global__ __launch_bounds__(256) void k_bc_syrk(double *__restrict__ A, int n, int k0,
+                                                 const double *__restrict__ scal) {
+    __shared__ double Li[BC][BC + 1], Lj[BC][BC + 1];
+    if (blockIdx.y > blockIdx.x || scal[kScalSolveOK] != 1.0) return;
+    const int kb = min(BC, n - k0), k1 = k0 + kb;
+    const int i0 = k1 + blockIdx.x * BC, j0 = k1 + blockIdx.y * BC;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int r = e / BC, p = e % BC;
+        Li[r][p] = (i0 + r < n && p < kb) ? A[(long)(i0 + r) * n + k0 + p] : 0.0;
+        Lj[r][p] = (j0 + r < n && p < kb) ? A[(long)(j0 + r) * n + k0 + p] : 0.0;
+    }
+    __syncthreads();
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int p = 0; p < kb; ++p) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = Li[ty + 16 * a][p];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = Lj[tx + 16 * b][p];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            if (i < n && j <= i) A[(long)i * n + j] -= acc[a][b];
+        }
+}
+
+// forward substitution of block k: X[k0..k1) = L_kk^-1 X[k0..k1), one
+// right-hand side per thread
+__global__ __launch_bounds__(256) void k_bc_fwd(const double *__restrict__ A, int n, int k0,
+                                                double *__restrict__ X, int nrhs,
+                                                const double *__restrict__ scal) {
+    __shared__ double L[BC][BC + 1];
+    if (scal[kScalSolveOK] != 1.0) return;
+    const int tid = threadIdx.x;
+    const int kb = min(BC, n - k0);
+    for (int e = tid; e < kb * kb; e += 256) L[e / kb][e % kb] = A[(long)(k0 + e / kb) * n + k0 + e % kb];
+    __syncthreads();
+    const int c = blockIdx.x * 256 + tid;
+    if (c >= nrhs) return;
+    double x[BC];
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+        if (j < kb) {
+            double s = X[(long)(k0 + j) * nrhs + c];
+#pragma unroll
+            for (int p = 0; p < j; ++p) s = fma(-L[j][p], x[p], s);
+            x[j] = s / L[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BC; ++j)
+        if (j < kb) X[(long)(k0 + j) * nrhs + c] = x[j];
+}
+
+// backward substitution of block k: X[k0..k1) = L_kk^-T X[k0..k1)
+__global__ __launch_bounds__(256) void k_bc_bwd(const double *__restrict__ A, int n, int k0,
+                                                double *__restrict__ X, int nrhs,
+                                                const double *__restrict__ scal) {
+    __shared__ double L[BC][BC + 1];
+    if (scal[kScalSolveOK] != 1.0) return;
+    const int tid = threadIdx.x;
+    const int kb = min(BC, n - k0);
+    for (int e = tid; e < kb * kb; e += 256) L[e / kb][e % kb] = A[(long)(k0 + e / kb) * n + k0 + e % kb];
+    __syncthreads();
+    const int c = blockIdx.x * 256 + tid;
+    if (c >= nrhs) return;
+    double x[BC];
+#pragma unroll
+    for (int jj = BC - 1; jj >= 0; --jj) {
+        if (jj < kb) {
+            double s = X[(long)(k0 + jj) * nrhs + c];
+#pragma unroll
+            for (int p = jj + 1; p < BC; ++p)
+                if (p < kb) s = fma(-L[p][jj], x[p], s);
+            x[jj] = s / L[jj][jj];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BC; ++j)
+        if (j < kb) X[(long)(k0 + j) * nrhs + c] = x[j];
+}
+
+// X[i] -= sum_p L[i][k0 + p] X[k0 + p] for the rows i >= k1 (forward) or
+// X[i] -= sum_p L[k0 + p][i] X[k0 + p] for the rows i < k0 (backward), in
+// 64 x 64 tiles of (rows, right-hand sides)
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_bc_update(const double *__restrict__ A, int n, int k0,
+                                                   double *__restrict__ X, int nrhs,
+                                                   const double *__restrict__ scal) {
+    __shared__ double Lt[BC][BC + 1], Xs[BC][BC + 1];
+    if (scal[kScalSolveOK] != 1.0) return;
+    const int kb = min(BC, n - k0), k1 = k0 + kb;
+    const int r0 = (FWD ? k1 : 0) + blockIdx.x * BC, c0 = blockIdx.y * BC;
+    const int rend = FWD ? n : k0;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int r = e / BC, p = e % BC;
+        const int i = r0 + r;
+        Lt[r][p] = (i < rend && p < kb)
+                       ? (FWD ? A[(long)i * n + k0 + p] : A[(long)(k0 + p) * n + i]) : 0.0;
+        Xs[r][p] = (r < kb && c0 + p < nrhs) ? X[(long)(k0 + r) * nrhs + c0 + p] : 0.0;
+    }
+    __syncthreads();
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int p = 0; p < kb; ++p) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = Lt[ty + 16 * a][p];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = Xs[p][tx + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = r0 + ty + 16 * a, c = c0 + tx + 16 * b;
+            if (i < rend && c < nrhs) X[(long)i * nrhs + c] -= acc[a][b];
+        }
+}
+
+// tr(A0) into scal[kScalJitter] (the retries' jitter scale)
+__global__ __launch_bounds__(64) void k_bc_trace(const double *__restrict__ A0, int n,
+                                                 double *__restrict__ scal) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) s += A0[(long)i * n + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (threadIdx.x == 0) scal[kScalJitter] = s;
+}
+
 // Tiled FP64 products of the M-step: 64 x 64 output tiles, 256 threads with
 // 4 x 4 results each (rows ty + 16 a, columns tx + 16 b: conflict-free LDS
 // reads), the inner dimension in steps of 16 staged through LDS.
@@ -168,13 +420,13 @@ constexpr int GT = 64, GK = 16;
 // Split-K partials of [A | B] = FI^T [diag(G) FI | RT], an M1 x (M1 + D)
 // product over the centres: workgroup (bx, by, c) computes one output tile
 // over centres [c kc, min(K, (c + 1) kc)); part is nch x M1 x (M1 + D).
-__global__ __launch_bounds__(256) void k_partAB(const double *__restrict__ FI,
-                                                const double *__restrict__ G,
-                                                const double *__restrict__ RT, int K, int M1,
-                                                int D, int kc, double *__restrict__ part) {
+__device__ __forceinline__ void k_partAB_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ FI,
+    const double *__restrict__ G, const double *__restrict__ RT, int K, int M1, int D, int kc,
+    double *__restrict__ part) {
     __shared__ double As[GK][GT], Bs[GK][GT];
     const int NCOL = M1 + D;
-    const int i0 = blockIdx.x * GT, j0 = blockIdx.y * GT, c = blockIdx.z;
+    const int i0 = bx * GT, j0 = by * GT, c = bz;
     const int k0 = c * kc, k1 = (k0 + kc < K) ? k0 + kc : K;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     double acc[4][4];
@@ -220,18 +472,31 @@ __global__ __launch_bounds__(256) void k_partAB(const double *__restrict__ FI,
         }
 }
 
+__global__ __launch_bounds__(256) void k_partAB(const double *__restrict__ FI,
+                                                const double *__restrict__ G,
+                                                const double *__restrict__ RT, int K, int M1,
+                                                int D, int kc, double *__restrict__ part) {
+    k_partAB_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, FI, G, RT, K, M1, D, kc, part);
+}
+
 // A0 / B from the chunk partials, summed in chunk order.
-__global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
-                        double *__restrict__ A0, double *__restrict__ B) {
+__device__ __forceinline__ void k_sumAB_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ part,
+    int nch, int M1, int D, double *__restrict__ A0, double *__restrict__ B) {
     const int NCOL = M1 + D;
     const long E = (long)M1 * NCOL;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long e = (long)bx * blockDim.x + threadIdx.x;
     if (e >= E) return;
     double s = 0.0;
     for (int c = 0; c < nch; ++c) s += part[(long)c * E + e];
     const int i = (int)(e / NCOL), j = (int)(e % NCOL);
     if (j < M1) A0[(long)i * M1 + j] = s;
     else B[(long)i * D + (j - M1)] = s;
+}
+
+__global__ void k_sumAB(const double *__restrict__ part, int nch, int M1, int D,
+                        double *__restrict__ A0, double *__restrict__ B) {
+    k_sumAB_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, part, nch, M1, D, A0, B);
 }
 
 constexpr int kLdsN = 126;   // largest M+1 whose factor fits in LDS
@@ -255,9 +520,9 @@ constexpr int kPB = 16;       // panel width of the blocked Cholesky
 // then all waves apply the panel to the trailing lower triangle with 4 x 4
 // register tiles.  Two barriers per panel.  Writes the factor's lower
 // triangle (the rest of L is left unspecified).
-__global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
-                                                   double *__restrict__ L, int n, double ridge,
-                                                   double *__restrict__ scal) {
+__device__ __forceinline__ void k_chol_lds_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ A0,
+    double *__restrict__ L, int n, double ridge, double *__restrict__ scal) {
     __shared__ double As[kLdsN * kLdsN];
     __shared__ double Pt[kPB][128];                  // the factored panel, transposed
     __shared__ double tr;
@@ -367,22 +632,28 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
     for (int e = tid; e < n * n; e += nt) L[e] = As[e];
 }
 
+__global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0,
+                                                   double *__restrict__ L, int n, double ridge,
+                                                   double *__restrict__ scal) {
+    k_chol_lds_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, A0, L, n, ridge, scal);
+}
+
 // W[:, c0:c0+32] from L y = b, L^T x = y (n <= 126): L in LDS, thread
 // (c, pr) of 32 x 32 keeps rows p = pr + 32 a of right-hand side c in
 // registers.  Step i: the owner of row i finishes x_i and publishes it in an
 // LDS ping-pong buffer; after one barrier every thread updates its rows after
 // (forward) or before (backward) row i.
-__global__ __launch_bounds__(1024) void k_trsm_lds(const double *__restrict__ L, int n,
-                                                   const double *__restrict__ B, int nrhs,
-                                                   double *__restrict__ W,
-                                                   const double *__restrict__ scal) {
+__device__ __forceinline__ void k_trsm_lds_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ L, int n,
+    const double *__restrict__ B, int nrhs, double *__restrict__ W,
+    const double *__restrict__ scal) {
     if (scal[kScalSolveOK] != 1.0) return;
     __shared__ double Ls[kLdsN * kLdsN];
     __shared__ double rd[kLdsN];
     __shared__ double xb[2][32];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int c = tid & 31, pr = tid >> 5;
-    const int c0 = blockIdx.x * 32;
+    const int c0 = bx * 32;
     const bool live = c0 + c < nrhs;
     for (int e = tid; e < n * n; e += nt) {
         const int i = e / n, k = e % n;
@@ -442,13 +713,20 @@ __global__ __launch_bounds__(1024) void k_trsm_lds(const double *__restrict__ L,
     }
 }
 
+__global__ __launch_bounds__(1024) void k_trsm_lds(const double *__restrict__ L, int n,
+                                                   const double *__restrict__ B, int nrhs,
+                                                   double *__restrict__ W,
+                                                   const double *__restrict__ scal) {
+    k_trsm_lds_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, L, n, B, nrhs, W, scal);
+}
+
 // Y = FI W into the padded FP64 layout (K x ld, columns >= D untouched):
 // 64 x 64 output tiles over (centre, d), the M1 inner dimension in steps of 16.
-__global__ __launch_bounds__(256) void k_phiw(const double *__restrict__ FI,
-                                              const double *__restrict__ W, int K, int M1, int D,
-                                              int ld, double *__restrict__ Y) {
+__device__ __forceinline__ void k_phiw_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ FI,
+    const double *__restrict__ W, int K, int M1, int D, int ld, double *__restrict__ Y) {
     __shared__ double As[GK][GT + 1], Bs[GK][GT];
-    const int k0 = blockIdx.x * GT, d0 = blockIdx.y * GT;
+    const int k0 = bx * GT, d0 = by * GT;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     double acc[4][4];
 #pragma unroll
@@ -488,20 +766,25 @@ __global__ __launch_bounds__(256) void k_phiw(const double *__restrict__ FI,
         }
 }
 
+__global__ __launch_bounds__(256) void k_phiw(const double *__restrict__ FI,
+                                              const double *__restrict__ W, int K, int M1, int D,
+                                              int ld, double *__restrict__ Y) {
+    k_phiw_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, FI, W, K, M1, D, ld, Y);
+}
+
 // Per-workgroup partials of sum_k (G_k |y_k|^2 - 2 y_k . RT_k) over 64
 // centres each (one wave per centre, 16 centres per wave); k_beta_final sums
 // them in workgroup order into scal[kScalBetaSum].
 constexpr int kBetaWG = 64;
-__global__ __launch_bounds__(256) void k_beta_part(const double *__restrict__ Y, int ld,
-                                                   const double *__restrict__ yy,
-                                                   const double *__restrict__ G,
-                                                   const double *__restrict__ RT, int K, int D,
-                                                   double *__restrict__ part) {
+__device__ __forceinline__ void k_beta_part_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ Y,
+    int ld, const double *__restrict__ yy, const double *__restrict__ G,
+    const double *__restrict__ RT, int K, int D, double *__restrict__ part) {
     __shared__ double s[4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     double acc = 0.0;                              // lane 0: this wave's sum
     for (int q = 0; q < kBetaWG / 4; ++q) {
-        const int k = blockIdx.x * kBetaWG + w * (kBetaWG / 4) + q;
+        const int k = bx * kBetaWG + w * (kBetaWG / 4) + q;
         if (k >= K) break;
         double dot = 0.0;
         for (int d = l; d < D; d += 64) dot = fma(Y[(long)k * ld + d], RT[(long)k * D + d], dot);
@@ -511,11 +794,20 @@ __global__ __launch_bounds__(256) void k_beta_part(const double *__restrict__ Y,
     }
     if (l == 0) s[w] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+    if (threadIdx.x == 0) part[bx] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-__global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ part, int n,
-                                                    double *__restrict__ scal) {
+__global__ __launch_bounds__(256) void k_beta_part(const double *__restrict__ Y, int ld,
+                                                   const double *__restrict__ yy,
+                                                   const double *__restrict__ G,
+                                                   const double *__restrict__ RT, int K, int D,
+                                                   double *__restrict__ part) {
+    k_beta_part_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, Y, ld, yy, G, RT, K, D, part);
+}
+
+__device__ __forceinline__ void k_beta_final_body(
+    const int bx, const int by, const int bz, const int gx, const double *__restrict__ part,
+    int n, double *__restrict__ scal) {
     __shared__ double s[256];
     double a = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) a += part[i];
@@ -526,6 +818,91 @@ __global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) scal[kScalBetaSum] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ part, int n,
+                                                    double *__restrict__ scal) {
+    k_beta_final_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, part, n, scal);
+}
+
+// ---------------------------------------------------------------- batched
+// The M-step of many instances per launch: tab holds one BInst per instance,
+// list the table indices of the instances in this launch, and one grid
+// dimension selects list[slot].  Each instance keeps its own grid extents;
+// workgroups past them return before any barrier.  Every instance computes
+// exactly what its own single-instance launch computes.
+
+__global__ __launch_bounds__(256) void kb_copy_g(const BInst *__restrict__ tab,
+                                                 const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (I.do_m && k < I.K) I.Gm[k] = I.G[k];
+}
+
+constexpr int kBatchNchMax = 64;   // centre chunks of the A/B reduction, at most
+
+__global__ __launch_bounds__(256) void kb_partAB(const BInst *__restrict__ tab,
+                                                 const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.z / kBatchNchMax]];
+    const int c = blockIdx.z % kBatchNchMax;
+    if (!I.do_m || c >= I.nchAB || (int)blockIdx.x * GT >= I.M1 || (int)blockIdx.y * GT >= I.M1 + I.D)
+        return;
+    const int kc = (I.K + I.nchAB - 1) / I.nchAB;
+    k_partAB_body(blockIdx.x, blockIdx.y, c, 0, I.FI, I.G, I.RT, I.K, I.M1, I.D, kc, I.partAB);
+}
+
+__global__ void kb_sumAB(const BInst *__restrict__ tab, const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_m || (long)blockIdx.x * 256 >= (long)I.M1 * I.M1 + (long)I.M1 * I.D) return;
+    k_sumAB_body(blockIdx.x, 0, 0, 0, I.partAB, I.nchAB, I.M1, I.D, I.A0, I.Bm);
+}
+
+__global__ __launch_bounds__(1024) void kb_chol_lds(const BInst *__restrict__ tab,
+                                                    const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.x]];
+    if (!I.do_m) return;
+    k_chol_lds_body(0, 0, 0, 1, I.A0, I.A, I.M1, I.ridge, I.scal);
+}
+
+__global__ __launch_bounds__(1024) void kb_trsm_lds(const BInst *__restrict__ tab,
+                                                    const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_m || (int)blockIdx.x * 32 >= I.D) return;
+    k_trsm_lds_body(blockIdx.x, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
+}
+
+__global__ __launch_bounds__(1024) void kb_chol_solve(const BInst *__restrict__ tab,
+                                                      const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.x]];
+    if (!I.do_m) return;
+    k_chol_solve_body(0, 0, 0, 1, I.A0, I.A, I.M1, I.ridge, I.Bm, I.D, I.W, I.scal);
+}
+
+__global__ __launch_bounds__(256) void kb_phiw(const BInst *__restrict__ tab,
+                                               const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.z]];
+    if (!I.do_m || (int)blockIdx.x * GT >= I.K || (int)blockIdx.y * GT >= I.D) return;
+    k_phiw_body(blockIdx.x, blockIdx.y, 0, 0, I.FI, I.W, I.K, I.M1, I.D, I.ldt, I.Y64);
+}
+
+__global__ void kb_rownorm_y(const BInst *__restrict__ tab, const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_m || (long)blockIdx.x * 256 >= I.K) return;
+    k_rownorm_body(blockIdx.x, 0, 0, 0, I.Y64, I.K, I.ldt, I.D, I.yy);
+}
+
+__global__ __launch_bounds__(256) void kb_beta_part(const BInst *__restrict__ tab,
+                                                    const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.y]];
+    if (!I.do_m || (int)blockIdx.x >= (I.K + kBetaWG - 1) / kBetaWG) return;
+    k_beta_part_body(blockIdx.x, 0, 0, 0, I.Y64, I.ldt, I.yy, I.G, I.RT, I.K, I.D, I.partAB);
+}
+
+__global__ __launch_bounds__(256) void kb_beta_final(const BInst *__restrict__ tab,
+                                                     const int *__restrict__ list) {
+    const BInst &I = tab[list[blockIdx.x]];
+    if (!I.do_m) return;
+    k_beta_final_body(0, 0, 0, 1, I.partAB, (I.K + kBetaWG - 1) / kBetaWG, I.scal);
 }
 
 }  // namespace
@@ -580,7 +957,53 @@ int dg_centres_from_w(dg_ctx *c) {
     return dg_rownorms(c, c->Y64, K, c->ldt, D, c->yy);
 }
 
-int dg_mstep_run(dg_ctx *c) {
+// (A0 + (ridge + jit) I) W = Bm for M + 1 > 126 by the blocked kernels;
+// clears scal[kScalSolveOK] on a non-positive pivot
+int blocked_chol_solve(dg_ctx *c, double ridge, double jit) {
+    const int n = c->M1, D = c->D;
+    const long tot = (long)n * n + (long)n * D;
+    hipLaunchKernelGGL(k_bc_init, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       c->A0, c->A, n, ridge, jit, c->Bm, c->W, D, c->scal);
+    const int nb = (n + BC - 1) / BC;
+    for (int k = 0; k < nb; ++k) {
+        const int k0 = k * BC, rest = n - k0 - BC;
+        hipLaunchKernelGGL(k_bc_potrf, dim3(1), dim3(256), 0, c->stream, c->A, n, k0, c->scal);
+        if (rest > 0) {
+            const int tb = (rest + BC - 1) / BC;
+            hipLaunchKernelGGL(k_bc_trsm, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0,
+                               c->stream, c->A, n, k0, c->scal);
+            hipLaunchKernelGGL(k_bc_syrk, dim3((unsigned)tb, (unsigned)tb), dim3(256), 0, c->stream,
+                               c->A, n, k0, c->scal);
+        }
+    }
+    const unsigned rb = (unsigned)((D + BC - 1) / BC);
+    for (int k = 0; k < nb; ++k) {
+        const int k0 = k * BC, rest = n - k0 - BC;
+        hipLaunchKernelGGL(k_bc_fwd, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c->stream,
+                           c->A, n, k0, c->W, D, c->scal);
+        if (rest > 0)
+            hipLaunchKernelGGL((k_bc_update<true>), dim3((unsigned)((rest + BC - 1) / BC), rb),
+                               dim3(256), 0, c->stream, c->A, n, k0, c->W, D, c->scal);
+    }
+    for (int k = nb - 1; k >= 0; --k) {
+        const int k0 = k * BC;
+        hipLaunchKernelGGL(k_bc_bwd, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c->stream,
+                           c->A, n, k0, c->W, D, c->scal);
+        if (k0 > 0)
+            hipLaunchKernelGGL((k_bc_update<false>), dim3((unsigned)((k0 + BC - 1) / BC), rb),
+                               dim3(256), 0, c->stream, c->A, n, k0, c->W, D, c->scal);
+    }
+    DG_HIP(c, hipGetLastError());
+    return DG_OK;
+}
+
+int dg_mstep_trace(dg_ctx *c) {
+    hipLaunchKernelGGL(k_bc_trace, dim3(1), dim3(64), 0, c->stream, c->A0, c->M1, c->scal);
+    DG_HIP(c, hipGetLastError());
+    return DG_OK;
+}
+
+int dg_mstep_run(dg_ctx *c, double jitter) {
     const int M1 = c->M1, D = c->D, K = c->K;
     const long E = (long)M1 * M1 + (long)M1 * D;
     const int nch = c->nchAB;
@@ -601,8 +1024,8 @@ int dg_mstep_run(dg_ctx *c) {
         hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + 31) / 32)), dim3(1024), 0, c->stream,
                            c->A, M1, c->Bm, D, c->W, c->scal);
     } else {
-        hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, c->stream, c->A0, c->A, M1,
-                           ridge, c->Bm, D, c->W, c->scal);
+        int rc = blocked_chol_solve(c, ridge, jitter);
+        if (rc) return rc;
     }
     DG_HIP(c, hipGetLastError());
     int rc = dg_centres_from_w(c);
@@ -615,4 +1038,39 @@ int dg_mstep_run(dg_ctx *c) {
     hipLaunchKernelGGL(k_beta_final, dim3(1), dim3(256), 0, c->stream, c->partAB, nb, c->scal);
     DG_HIP(c, hipGetLastError());
     return DG_OK;
+}
+
+// M-step of the instances in list (n of them; nlds with M + 1 <= 126 first in
+// list, the others after), on one stream.  mx holds the largest K, D, M + 1
+// over them.  Each instance's steps are those of dg_mstep_run after the copy
+// of G into Gm.
+int dg_batch_mstep(hipStream_t st, const BInst *tab, const int *list, int n, int nlds,
+                   const BShape &mx) {
+    if (n <= 0) return DG_OK;
+    const unsigned un = (unsigned)n;
+    hipLaunchKernelGGL(kb_copy_g, dim3((unsigned)((mx.K + 255) / 256), un), dim3(256), 0, st, tab,
+                       list);
+    hipLaunchKernelGGL(kb_partAB,
+                       dim3((unsigned)((mx.M1 + GT - 1) / GT), (unsigned)((mx.M1 + mx.D + GT - 1) / GT),
+                            un * kBatchNchMax),
+                       dim3(256), 0, st, tab, list);
+    const long E = (long)mx.M1 * mx.M1 + (long)mx.M1 * mx.D;
+    hipLaunchKernelGGL(kb_sumAB, dim3((unsigned)((E + 255) / 256), un), dim3(256), 0, st, tab, list);
+    if (nlds > 0) {
+        hipLaunchKernelGGL(kb_chol_lds, dim3((unsigned)nlds), dim3(1024), 0, st, tab, list);
+        hipLaunchKernelGGL(kb_trsm_lds, dim3((unsigned)((mx.D + 31) / 32), (unsigned)nlds),
+                           dim3(1024), 0, st, tab, list);
+    }
+    if (n > nlds)
+        hipLaunchKernelGGL(kb_chol_solve, dim3((unsigned)(n - nlds)), dim3(1024), 0, st, tab,
+                           list + nlds);
+    hipLaunchKernelGGL(kb_phiw, dim3((unsigned)((mx.K + GT - 1) / GT), (unsigned)((mx.D + GT - 1) / GT), un),
+                       dim3(256), 0, st, tab, list);
+    hipLaunchKernelGGL(kb_rownorm_y, dim3((unsigned)((mx.K + 255) / 256), un), dim3(256), 0, st, tab,
+                       list);
+    hipLaunchKernelGGL(kb_beta_part, dim3((unsigned)((mx.K + kBetaWG - 1) / kBetaWG), un), dim3(256),
+                       0, st, tab, list);
+    hipLaunchKernelGGL(kb_beta_final, dim3(un), dim3(256), 0, st, tab, list);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DG_OK : DG_EHIP;
 }

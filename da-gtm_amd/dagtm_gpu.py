@@ -341,8 +341,10 @@ class Engine:
         return list(t)
 
 
-def batch_anneal(problems, precision=FP64, lam=0.0, workers=4, device=0, **anneal_kw):
-    """Anneal independent instances concurrently (dg_batch_anneal).
+def batch_anneal(problems, precision=FP64, lam=0.0, workers=0, device=0, **anneal_kw):
+    """Anneal independent instances concurrently (dg_batch_anneal).  workers:
+    0 = batched launches replayed as hipGraphs (mixed precisions), -1 = the
+    same without graphs, 1..4 = host worker threads.
 
     problems: list of (T, FI, Y0, beta0) or (T, FI, Y0, beta0, params_dict); the
     keyword arguments are the default annealing parameters (as Engine.anneal).

@@ -10,6 +10,8 @@ fingerprints(N, D, ...)    binary 0/1 fingerprint-like vectors: P prototypes
 latent_basis(L, K, M, s)   latent grid X and RBF basis FI (+bias column) of
                            the reference's dagtm_stp (comm.c:992-1103,
                            1453-1536), written with numpy.
+batch_instances(n, seed)   the batched workload: n independent instances of
+                           uneven N, D, K (see the function).
 """
 import numpy as np
 
@@ -80,3 +82,25 @@ def pca_init(T, FI, X):
     var = ev[L] if L < len(ev) else ev[-1]
     beta0 = 1.0 / max(min(interdist, var), 1e-12) if np.isfinite(interdist) else 1.0 / max(var, 1e-12)
     return W0, Y0, float(beta0)
+
+
+def batch_instances(n=64, seed=2024):
+    """n independent instances: N log-uniform in [1e3, 2e5]; D from {12, 32,
+    64, 166} (Gaussian-mixture data for D <= 64, binary fingerprints for 166);
+    (L, K, M) from {(2, 400, 25), (2, 1600, 25), (3, 8000, 125)}; independent
+    seeds and start temperatures in [1.5, 4).  Returns a list of (T, FI, Y0,
+    beta0, {"start_temp": t0})."""
+    rng = np.random.default_rng(seed)
+    shapes = [(2, 400, 25), (2, 1600, 25), (3, 8000, 125)]
+    out = []
+    for _ in range(n):
+        N = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        D = int(rng.choice([12, 32, 64, 166]))
+        L, K, M = shapes[int(rng.integers(0, 3))]
+        s = int(rng.integers(1, 1 << 30))
+        T = (fingerprints(N, D, s).astype(np.float64) if D == 166 else gmm(N, D, s))
+        X, FI = latent_basis(L, K, M, 1.0)
+        W0, Y0, b0 = pca_init(T, FI, X)
+        out.append((T, FI, Y0, b0, dict(start_temp=float(rng.uniform(1.5, 4.0)))))
+    return out
+

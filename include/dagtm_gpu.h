@@ -211,12 +211,21 @@ int dg_next_ctemp(dg_ctx *ctx, double temp, double eps, const double *gg,
 
 /* ---- many independent instances ------------------------------------------
  * One annealing run per item (any N, D, K, M per item), on one GPU.  Every
- * item gets its own context (and HIP stream) with its data resident on the
- * device; then a pool of `workers` host threads takes the items from a shared
- * queue in order of decreasing K*N*D and runs their annealing loops, so
- * kernels of different instances run concurrently.  Results do not depend on
- * the worker count or order (every kernel is deterministic).  The reference
- * runs one instance per process; this has no counterpart there. */
+ * item gets its own context with its data resident on the device and its own
+ * annealing control.  In the mixed precisions with workers = 0 every step of
+ * all the instances runs as one sequence of batched launches: each E-pass of
+ * a template class (D range, data terms, R terms) is one persistent launch
+ * over a device work queue holding the work items of every instance, and
+ * the other kernels take the instance from a grid dimension; the sequence of
+ * each phase (E-step, M-step) is captured once as a hipGraph and replayed
+ * every step, with per-instance parameters and on/off flags in a device
+ * table.  workers = -1 runs the same launches without graph capture.  With
+ * workers >= 1 (and in FP64) a pool of host threads takes the items from a
+ * shared queue in order of decreasing K*N*D and runs their annealing loops on
+ * their own streams.  Results do not depend on the mode, the worker count or
+ * the order: each item's result equals its own dg_anneal run bit for bit.
+ * The reference runs one instance per process; this has no counterpart
+ * there. */
 typedef struct {
     const double *T;  long N; int D; long ldT;     /* data, N x D */
     const double *FI; int K; int M1; long ldFI;    /* basis, K x M1 */
@@ -227,8 +236,8 @@ typedef struct {
     double *Y; long ldY;                           /* out: K x D, may be NULL */
 } dg_batch_item;
 
-/* cfg: device, precision and lambda shared by all items; workers is clamped
- * to 1..4 (<= 0 means 4).  res[i] and status[i] (DG_OK or the item's error code; a failing item
+/* cfg: device, precision and lambda shared by all items; workers: 0 or -1
+ * (above; in FP64 they mean 4), else clamped to 1..4.  res[i] and status[i] (DG_OK or the item's error code; a failing item
  * does not stop the others) receive each item's outcome; phase_ms (may be
  * NULL) the wall time of the upload phase and of the annealing phase.
  * Returns DG_OK, or DG_EINVAL / DG_EHIP when the batch could not start. */

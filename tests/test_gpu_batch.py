@@ -1,7 +1,9 @@
 """Many independent instances through dg_batch_anneal: every kernel is
 deterministic, so each instance's result must equal, bit for bit, the same
-instance annealed alone in one context -- whatever the worker count and the
-order in which the workers picked the instances up."""
+instance annealed alone in one context -- in the batched mode (all
+instances' steps as batched launches over a device work queue, replayed as
+hipGraphs, or launched directly) and with host workers, whatever order the
+work items or the instances are taken in."""
 import sys
 
 import numpy as np
@@ -36,7 +38,8 @@ def _instances():
     return out
 
 
-@pytest.mark.parametrize("prec,workers", [(dg.FP64, 3), (dg.MIXED, 4)])
+@pytest.mark.parametrize("prec,workers", [(dg.FP64, 3), (dg.MIXED, 4), (dg.MIXED, 0),
+                                          (dg.MIXED, -1), (dg.MIXED_R1, 0)])
 def test_batch_equals_sequential(prec, workers):
     _say(f"case: batch of {len(CASES)} instances, precision {prec}, {workers} workers")
     probs = _instances()
@@ -91,3 +94,65 @@ def test_tuned_chunk_count_is_used_from_the_table(tmp_path, monkeypatch):
         assert e.tune(None) == 0
         l64 = e.estep(True, 2.0)[0]
     assert got[0] == pytest.approx(l64, rel=1e-5)
+
+
+@pytest.fixture(scope="module")
+def c4():
+    _say("generating the 64-instance batched workload")
+    return datagen.batch_instances()
+
+
+def test_c4_batch_one_iteration(c4):
+    """The 64-instance workload (N log-uniform in [1e3, 2e5], D in {12, 32, 64,
+    166}, K in {400, 1600, 8000}), one DA E+M iteration per instance at its
+    start temperature in one batched step.  Each instance must equal its own
+    single-context run bit for bit.  Against FP64 -- the oracle where it can
+    hold the instance (N K <= 4e7), else the FP64 GPU path:
+      - binary data (exact in FP16): beta 1e-6, Y 1e-4 of max |Y|, and
+        sum_k G_k = N to 1e-6;
+      - Gaussian-mixture float data (both operands split): beta 5e-4, Y 1e-4,
+        sum_k G_k = N to 1e-4.  Their exponents reach a few thousand (log2
+        units: beta |t|^2 / 2 with |t|^2 up to ~300), so the FP32 roundings of
+        pass 1's normaliser and of pass 2's recomputed exponents differ by up
+        to ~1e-4 relative per element; measured on this workload: sum G 2e-5,
+        beta 1.7e-4, Y 2.7e-5 at worst."""
+    import oracle as O
+    O.set_threads(16)
+    out, ph = dg.batch_anneal(c4, precision=dg.MIXED, lam=1e-3, workers=0, alpha=0.99, eps=0.0,
+                              maxloop=1)
+    errs = []
+    for i, (p, (st, r, W, Y)) in enumerate(zip(c4, out)):
+        T, FI, Y0, b0, kw = p
+        t0 = kw["start_temp"]
+        assert st == dg.DG_OK and r.nloop == 1, (i, st)
+        with dg.Engine(precision=dg.MIXED, lam=1e-3) as e:
+            e.upload(T, FI, Y0, b0)
+            e.estep(True, t0)
+            G, _ = e.stats()
+            e.mstep()
+            Ws, Ys, bs = e.download()
+        assert np.array_equal(W, Ws) and np.array_equal(Y, Ys) and r.beta == bs, i
+        N, D, K = T.shape[0], T.shape[1], FI.shape[0]
+        if N * K <= 4e7:
+            m = O.Model(T, FI, Y0, b0, lam=1e-3, solver=O.SOLVE_CHOL)
+            m.estep(1, t0)
+            m.mstep()
+            bref, Yref = m.beta, m.Y
+            del m
+        else:
+            with dg.Engine(precision=dg.FP64, lam=1e-3) as e:
+                e.upload(T, FI, Y0, b0)
+                e.estep(True, t0)
+                bref = e.mstep()
+                _, Yref, _ = e.download()
+        eg = abs(G.sum() - N) / N
+        eb = abs(r.beta - bref) / bref
+        ey = float(np.abs(Y - Yref).max() / np.abs(Yref).max())
+        errs.append((i, N, D, K, eg, eb, ey))
+        _say(f"  instance {i}: N={N} D={D} K={K} sum G rel {eg:.1e} beta {eb:.1e} Y {ey:.1e}")
+    _say(f"  64 instances in one batched step ({ph[1]:.1f} ms)")
+    for (i, N, D, K, eg, eb, ey) in errs:
+        if D == 166:
+            assert eg <= 1e-6 and eb <= 1e-6 and ey <= 1e-4, (i, eg, eb, ey)
+        else:
+            assert eg <= 1e-4 and eb <= 5e-4 and ey <= 1e-4, (i, eg, eb, ey)

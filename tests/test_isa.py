@@ -80,8 +80,8 @@ def _kernels(lines, stem):
 
 
 def test_mix_split_is_fenced_from_mfma(asm):
-    ks = _kernels(asm, "k_e2_mfma")
-    assert len(ks) >= 32, "pass-2 instantiations not found"
+    ks = {**_kernels(asm, "k_e2_mfma"), **_kernels(asm, "kb_e2_mfma")}
+    assert len(ks) >= 64, "pass-2 instantiations not found"
     for name, body in ks.items():
         d = _min_mix_to_mfma(body)
         hilo = re.search(r"Lb[01]ELb1EE", name) is not None     # last template flag: R as hi/lo
@@ -103,36 +103,39 @@ def test_pass1_accumulator_in_vgprs(asm):
     VGPRs (no AGPRs, no v_accvgpr_read), and every instantiation must still fit
     three waves per SIMD (at most 168 registers)."""
     text = "\n".join(asm)
-    names = sorted(set(re.findall(r"^(_Z\w*k_e1_mfma\w*):", text, re.M)))
-    assert len(names) >= 16, "pass-1 instantiations not found"
+    names = sorted(set(re.findall(r"^(_Z\w*kb?_e1_mfma\w*):", text, re.M)))
+    assert len(names) >= 30, "pass-1 instantiations not found"
     for name in names:
         agpr = re.search(r"\.set " + name + r"\.num_agpr, (\d+)", text)
         vgpr = re.search(r"\.set " + name + r"\.num_vgpr, (\d+)", text)
         assert agpr and vgpr, name
         assert int(agpr.group(1)) == 0, (name, agpr.group(1))
         assert int(vgpr.group(1)) <= 168, (name, vgpr.group(1))
-    for name, body in _kernels(asm, "k_e1_mfma").items():
-        assert not any(x.startswith("v_accvgpr_read") for x in body), name
+    for stem in ("k_e1_mfma", "kb_e1_mfma"):
+        for name, body in _kernels(asm, stem).items():
+            assert not any(x.startswith("v_accvgpr_read") for x in body), name
 
 
 def test_dma_uses_scalar_base(asm):
     """Every LDS-DMA of the E-pass streams takes its address as a scalar base
-    plus a 32-bit lane offset (global_load_lds ... vN, s[a:b]): no per-load
+    plus a 32-bit lane offset (global_load_lds ... vN, s[a:b], or the SGPR pair
+    vcc): no per-load
     64-bit VGPR address, which would cost a VALU add per load."""
-    for stem in ("k_e1_mfma", "k_e2_mfma"):
+    for stem in ("k_e1_mfma", "k_e2_mfma", "kb_e1_mfma", "kb_e2_mfma"):
         ks = _kernels(asm, stem)
         assert ks, stem
         for name, body in ks.items():
             loads = [x for x in body if x.startswith("global_load_lds")]
             assert loads, name
-            bad = [x for x in loads if not re.match(r"global_load_lds_dwordx4 v\d+, s\[\d+:\d+\]", x)]
+            bad = [x for x in loads
+                   if not re.match(r"global_load_lds_dwordx4 v\d+, (s\[\d+:\d+\]|vcc)(,|\s|$)", x)]
             assert not bad, (name, bad[:2])
 
 
 def test_pass2_loop_has_no_vector_scalar_loads(asm):
     """The per-block mask-threshold read of pass 2 is a scalar load: at most
     the prologue's single global_load_dword remains in each instantiation."""
-    for name, body in _kernels(asm, "k_e2_mfma").items():
+    for name, body in {**_kernels(asm, "k_e2_mfma"), **_kernels(asm, "kb_e2_mfma")}.items():
         n = sum(1 for x in body if x.startswith("global_load_dword "))
         assert n <= 1, (name, n)
 
@@ -142,7 +145,7 @@ def test_pass1_loop_waits_for_dma_only_at_the_barrier(asm):
     LDS-DMA, so every vmcnt wait in each instantiation belongs to a barrier
     (prologue and loop end): no LDS read sits between such a wait and the
     next s_barrier."""
-    for name, body in _kernels(asm, "k_e1_mfma").items():
+    for name, body in {**_kernels(asm, "k_e1_mfma"), **_kernels(asm, "kb_e1_mfma")}.items():
         for j, x in enumerate(body):
             if x.startswith("s_waitcnt") and "vmcnt" in x:
                 nxt = next((y for y in body[j + 1:] if y.startswith(("s_barrier", "ds_read"))), None)
@@ -172,9 +175,22 @@ def test_pass2_reads_every_operand(asm):
     an operand load placed before the first cross-term MFMA of a phase with
     fewer MFMAs than the offset) leaves an operand register unread and the
     R T product reads stale data."""
-    for name, body in _kernels(asm, "k_e2_mfma").items():
-        m = re.search(r"k_e2_mfmaILi(\d+)ELi(\d+)ELb([01])ELb([01])E", name)
+    for name, body in {**_kernels(asm, "k_e2_mfma"), **_kernels(asm, "kb_e2_mfma")}.items():
+        m = re.search(r"kb?_e2_mfmaILi(\d+)ELi(\d+)ELb([01])ELb([01])E", name)
         assert m, name
         KS, TT = int(m.group(1)), int(m.group(2))
         n = sum(1 for x in body if x.startswith("ds_read_b128"))
         assert n == _e2_lds_reads_expected(KS, TT), (name, n, _e2_lds_reads_expected(KS, TT))
+
+
+def test_e_passes_do_not_spill(asm):
+    """No E-pass instantiation, single or batched (work-queue loop), uses
+    scratch memory: in the batched loop the bodies' lane-derived values are
+    recomputed per work item rather than held across it."""
+    text = "\n".join(asm)
+    names = set(re.findall(r"^(_Z\w*kb?_e[12]_mfma\w*):", text, re.M))
+    assert len(names) >= 90
+    for name in names:
+        ps = re.search(r"\.set " + re.escape(name) + r"\.private_seg_size, (\d+)", text)
+        assert ps and int(ps.group(1)) == 0, (name, ps and ps.group(1))
+
