@@ -4,10 +4,11 @@
 // as one sequence of batched launches on one stream:
 //
 //   E-phase: the instance table (parameters of this step, which instances
-//            take an E-step) goes to the device, then make_ak, pass 1 of every
-//            template class as one persistent launch over a work queue, the
-//            log sums, pass 2 likewise, the chunk reductions, the zero-column
-//            fix, and one copy of every instance's scalars to the host;
+//            take an E-step) goes to the device, then make_ak, and per
+//            template class -- the classes concurrently on side streams --
+//            pass 1 as one persistent launch over a work queue, the log sums,
+//            pass 2 likewise, the chunk reductions and the zero-column fix;
+//            then one copy of every instance's scalars to the host;
 //   M-phase: the table again (which instances take an M-step, their ridge),
 //            the M-step kernels of all instances per launch, the FP16 split of
 //            the new centres, and the scalars.
@@ -63,8 +64,16 @@ struct Batch {
     int nlds = 0;
     BShape mx;
     Graph ge, gm;
+    // the E-phase runs the template classes concurrently on side streams
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev[1 + kSide] = {};
 
     ~Batch() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t t : side)
+            if (t) (void)hipStreamDestroy(t);
         if (h_tab) (void)hipHostFree(h_tab);
         if (h_list) (void)hipHostFree(h_list);
         if (h_scal) (void)hipHostFree(h_scal);
@@ -92,7 +101,7 @@ int e_phase(Batch &b) {
     DG_HIP(nullptr, hipMemcpyAsync(b.d_tab, b.h_tab, sizeof(BInst) * b.n, hipMemcpyHostToDevice,
                                    b.st));
     int rc = dg_batch_estep(b.st, b.d_tab, b.d_list, b.n, b.cls.data(), (int)b.cls.size(),
-                            b.d_counters, b.mx, b.cus);
+                            b.d_counters, b.mx, b.cus, b.side, Batch::kSide, b.ev);
     if (rc) return rc;
     if ((rc = dg_batch_gather_scal(b.st, b.d_tab, b.n, b.d_scal))) return rc;
     DG_HIP(nullptr, hipMemcpyAsync(b.h_scal, b.d_scal, sizeof(double) * kScalCount * b.n,
@@ -154,6 +163,8 @@ int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *param
     DG_HIP(nullptr, hipGetDevice(&dev));
     DG_HIP(nullptr, hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, dev));
     DG_HIP(nullptr, hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
+    for (hipStream_t &t : b.side) DG_HIP(nullptr, hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    for (hipEvent_t &e : b.ev) DG_HIP(nullptr, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     DG_HIP(nullptr, hipHostMalloc((void **)&b.h_tab, sizeof(BInst) * n, hipHostMallocDefault));
     DG_HIP(nullptr, hipHostMalloc((void **)&b.h_list, sizeof(int) * 2 * n, hipHostMallocDefault));
     DG_HIP(nullptr, hipHostMalloc((void **)&b.h_scal, sizeof(double) * kScalCount * n,

@@ -137,11 +137,13 @@ struct BClass {
 // batched E-step pieces (dg_mixed.hip)
 int dg_batch_fill_mixed(const dg_ctx *c, BInst *I);
 int dg_batch_items(const BInst &I, int *e1, int *e2);
-// make_ak, pass 1 per class, the log sums, pass 2 per class, the chunk
-// reductions and the zero-column fix of the n listed instances; counters
-// holds 4 ncls ints of device memory
+// make_ak of the n listed instances on st; then per class, on side stream
+// (class index) % nside: pass 1, the log sums, pass 2, the chunk reductions
+// and the zero-column fix; st waits for every side stream at the end.
+// counters holds 4 ncls ints of device memory, ev 1 + nside events.
 int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
-                   int ncls, int *counters, const BShape &mx, int cus);
+                   int ncls, int *counters, const BShape &mx, int cus, hipStream_t *side,
+                   int nside, hipEvent_t *ev);
 // the FP16 split of the new centres (dg_mixed_prepare_y) of the listed instances
 int dg_batch_prepare_y(hipStream_t st, const BInst *tab, const int *list, int n,
                        const BShape &mx);

@@ -1607,28 +1607,40 @@ int dg_batch_items(const BInst &I, int *e1, int *e2) {
 }
 
 int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
-                   int ncls, int *counters, const BShape &mx, int cus) {
+                   int ncls, int *counters, const BShape &mx, int cus, hipStream_t *side,
+                   int nside, hipEvent_t *ev) {
     if (n <= 0) return DG_OK;
-    const unsigned un = (unsigned)n;
     hipLaunchKernelGGL(kb_zero_ints, dim3((unsigned)((4 * ncls + 255) / 256)), dim3(256), 0, st,
                        counters, 4 * ncls);
-    hipLaunchKernelGGL(kb_make_ak, dim3((unsigned)((mx.Kp + 255) / 256), un), dim3(256), 0, st,
-                       tab, list);
+    hipLaunchKernelGGL(kb_make_ak, dim3((unsigned)((mx.Kp + 255) / 256), (unsigned)n), dim3(256), 0,
+                       st, tab, list);
+    // the classes are independent until the scalars are gathered: class i runs
+    // on side stream i % nside (fork after make_ak, join before the gather)
+    if (hipEventRecord(ev[0], st) != hipSuccess) return DG_EHIP;
+    for (int j = 0; j < nside; ++j)
+        if (hipStreamWaitEvent(side[j], ev[0], 0) != hipSuccess) return DG_EHIP;
     int rc;
     for (int i = 0; i < ncls; ++i) {
-        rc = (cls[i].TT == 1) ? launch_class<1>(cls[i], st, tab, list, counters, false, cus)
-                              : launch_class<2>(cls[i], st, tab, list, counters, false, cus);
+        const BClass &k = cls[i];
+        hipStream_t ss = side[i % nside];
+        const int *l = list + k.first;
+        const unsigned un = (unsigned)k.n;
+        rc = (k.TT == 1) ? launch_class<1>(k, ss, tab, list, counters, false, cus)
+                         : launch_class<2>(k, ss, tab, list, counters, false, cus);
         if (rc) return rc;
-    }
-    hipLaunchKernelGGL(kb_sum_partlog, dim3(un), dim3(1024), 0, st, tab, list);
-    for (int i = 0; i < ncls; ++i) {
-        rc = (cls[i].TT == 1) ? launch_class<1>(cls[i], st, tab, list, counters, true, cus)
-                              : launch_class<2>(cls[i], st, tab, list, counters, true, cus);
+        hipLaunchKernelGGL(kb_sum_partlog, dim3(un), dim3(1024), 0, ss, tab, l);
+        rc = (k.TT == 1) ? launch_class<1>(k, ss, tab, list, counters, true, cus)
+                         : launch_class<2>(k, ss, tab, list, counters, true, cus);
         if (rc) return rc;
+        hipLaunchKernelGGL(kb_reduce32,
+                           dim3((unsigned)((mx.K + RK - 1) / RK), (unsigned)((mx.D + RD - 1) / RD), un),
+                           dim3(RK * RD), 0, ss, tab, l);
+        hipLaunchKernelGGL(kb_zero_fix, dim3((unsigned)mx.D, un), dim3(256), 0, ss, tab, l);
     }
-    hipLaunchKernelGGL(kb_reduce32, dim3((unsigned)((mx.K + RK - 1) / RK), (unsigned)((mx.D + RD - 1) / RD), un),
-                       dim3(RK * RD), 0, st, tab, list);
-    hipLaunchKernelGGL(kb_zero_fix, dim3((unsigned)mx.D, un), dim3(256), 0, st, tab, list);
+    for (int j = 0; j < nside; ++j) {
+        if (hipEventRecord(ev[1 + j], side[j]) != hipSuccess) return DG_EHIP;
+        if (hipStreamWaitEvent(st, ev[1 + j], 0) != hipSuccess) return DG_EHIP;
+    }
     return hipGetLastError() == hipSuccess ? DG_OK : DG_EHIP;
 }
 
