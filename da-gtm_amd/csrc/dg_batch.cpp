@@ -62,7 +62,8 @@ struct Batch {
     double *h_scal = nullptr, *d_scal = nullptr;
     std::vector<BClass> cls;
     int nlds = 0;
-    BShape mx;
+    BGrids gr;
+    int2 *d_map = nullptr;     // the flat-grid maps (BGrids, BClass)
     Graph ge, gm;
     // the E-phase runs the template classes concurrently on side streams
     static constexpr int kSide = 4;
@@ -80,6 +81,7 @@ struct Batch {
         if (d_tab) (void)hipFree(d_tab);
         if (d_list) (void)hipFree(d_list);
         if (d_counters) (void)hipFree(d_counters);
+        if (d_map) (void)hipFree(d_map);
         if (d_scal) (void)hipFree(d_scal);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -101,7 +103,7 @@ int e_phase(Batch &b) {
     DG_HIP(nullptr, hipMemcpyAsync(b.d_tab, b.h_tab, sizeof(BInst) * b.n, hipMemcpyHostToDevice,
                                    b.st));
     int rc = dg_batch_estep(b.st, b.d_tab, b.d_list, b.n, b.cls.data(), (int)b.cls.size(),
-                            b.d_counters, b.mx, b.cus, b.side, Batch::kSide, b.ev);
+                            b.d_counters, b.gr, b.cus, b.side, Batch::kSide, b.ev);
     if (rc) return rc;
     if ((rc = dg_batch_gather_scal(b.st, b.d_tab, b.n, b.d_scal))) return rc;
     DG_HIP(nullptr, hipMemcpyAsync(b.h_scal, b.d_scal, sizeof(double) * kScalCount * b.n,
@@ -112,9 +114,9 @@ int e_phase(Batch &b) {
 int m_phase(Batch &b) {
     DG_HIP(nullptr, hipMemcpyAsync(b.d_tab, b.h_tab, sizeof(BInst) * b.n, hipMemcpyHostToDevice,
                                    b.st));
-    int rc = dg_batch_mstep(b.st, b.d_tab, b.d_list + b.n, b.n, b.nlds, b.mx);
+    int rc = dg_batch_mstep(b.st, b.d_tab, b.d_list + b.n, b.n, b.nlds, b.gr);
     if (rc) return rc;
-    if ((rc = dg_batch_prepare_y(b.st, b.d_tab, b.d_list, b.n, b.mx))) return rc;
+    if ((rc = dg_batch_prepare_y(b.st, b.d_tab, b.d_list, b.n, b.gr))) return rc;
     if ((rc = dg_batch_gather_scal(b.st, b.d_tab, b.n, b.d_scal))) return rc;
     DG_HIP(nullptr, hipMemcpyAsync(b.h_scal, b.d_scal, sizeof(double) * kScalCount * b.n,
                                    hipMemcpyDeviceToHost, b.st));
@@ -181,11 +183,6 @@ int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *param
         const int DR = 32 * ((16 * I.KS + 31) / 32);
         const int rlo = ctx[i]->cfg.precision == DG_MIXED ? 1 : 0;
         byclass[std::make_tuple(I.KS, I.TT, DR > I.D ? 1 : 0, rlo)].push_back(i);
-        b.mx.Np = std::max(b.mx.Np, I.Np);
-        b.mx.Kp = std::max(b.mx.Kp, (long)I.Kp);
-        b.mx.K = std::max(b.mx.K, I.K);
-        b.mx.D = std::max(b.mx.D, I.D);
-        b.mx.M1 = std::max(b.mx.M1, I.M1);
     }
     int pos = 0;
     for (auto &kv : byclass) {
@@ -217,6 +214,45 @@ int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *param
     b.nlds = q - n;
     for (int i = 0; i < n; ++i)
         if (b.h_tab[i].M1 > 126) b.h_list[q++] = i;
+    // flat-grid maps: E-list kinds, per-class kinds, M-list kinds
+    std::vector<int2> hmap;
+    auto add_map = [&](const int *l, int cnt, int kind) {
+        const int at = (int)hmap.size();
+        for (int j = 0; j < cnt; ++j) {
+            const int w = bk_count(b.h_tab[l[j]], kind);
+            for (int g = 0; g < w; ++g) hmap.push_back(make_int2(l[j], g));
+        }
+        return std::make_pair(at, (int)hmap.size() - at);
+    };
+    int at[BK_COUNT] = {};
+    std::vector<std::pair<int, int>> at_cls;
+    auto whole = [&](const int *l, int cnt, int kind) {
+        const auto r = add_map(l, cnt, kind);
+        at[kind] = r.first;
+        b.gr.wg[kind] = r.second;
+    };
+    whole(b.h_list, n, BK_MAKE_AK);
+    whole(b.h_list, n, BK_ABSMAX_PART);
+    whole(b.h_list, n, BK_MAKE_Y16);
+    for (BClass &k : b.cls) {
+        const auto r1 = add_map(b.h_list + k.first, k.n, BK_REDUCE32);
+        const auto r2 = add_map(b.h_list + k.first, k.n, BK_ZERO_FIX);
+        k.wg_reduce32 = r1.second;
+        k.wg_zero_fix = r2.second;
+        at_cls.push_back(r1);
+        at_cls.push_back(r2);
+    }
+    for (int kind : {BK_COPY_G, BK_PART_AB, BK_SUM_AB, BK_PHIW, BK_ROWNORM, BK_BETA_PART})
+        whole(b.h_list + n, n, kind);
+    whole(b.h_list + n, b.nlds, BK_TRSM_LDS);
+    DG_HIP(nullptr, hipMalloc((void **)&b.d_map, sizeof(int2) * std::max<size_t>(1, hmap.size())));
+    DG_HIP(nullptr, hipMemcpy(b.d_map, hmap.data(), sizeof(int2) * hmap.size(),
+                              hipMemcpyHostToDevice));
+    for (int kind = 0; kind < BK_COUNT; ++kind) b.gr.map[kind] = b.d_map + at[kind];
+    for (size_t i = 0; i < b.cls.size(); ++i) {
+        b.cls[i].map_reduce32 = b.d_map + at_cls[2 * i].first;
+        b.cls[i].map_zero_fix = b.d_map + at_cls[2 * i + 1].first;
+    }
     DG_HIP(nullptr, hipMalloc((void **)&b.d_counters, sizeof(int) * 4 * b.cls.size()));
     DG_HIP(nullptr, hipMemcpy(b.d_list, b.h_list, sizeof(int) * 2 * n, hipMemcpyHostToDevice));
     for (dg_ctx *c : ctx) DG_HIP(c, hipStreamSynchronize(c->stream));

@@ -97,6 +97,18 @@ enum {
 };
 
 // ---- batched instances ----------------------------------------------------
+// Batched light kernels run one flat grid over the instances of their list,
+// each instance with exactly its own workgroup count (bk_count): a device map
+// per launch gives workgroup g its (table index, workgroup within instance).
+enum BKind {
+    BK_MAKE_AK, BK_REDUCE32, BK_ZERO_FIX, BK_COPY_G, BK_PART_AB, BK_SUM_AB, BK_TRSM_LDS,
+    BK_PHIW, BK_ROWNORM, BK_BETA_PART, BK_ABSMAX_PART, BK_MAKE_Y16, BK_COUNT
+};
+// tile sizes the flat grids are counted in (checked against the kernels)
+constexpr int kBRK = 64, kBRD = 16;     // k_reduce32: centres x dims per workgroup
+constexpr int kBGT = 64;                 // M-step products: output tile
+constexpr int kBBetaWG = 64;             // k_beta_part: centres per workgroup
+constexpr int kBAmaxWG = 256;            // k_absmax_part: most workgroups
 // One instance of a batched step, as the batched kernels read it from a
 // device table: the shape, the device buffers of its context and the
 // parameters of the current step.  Built on the host from each context.
@@ -120,11 +132,30 @@ struct BInst {
     int item_e1, item_e2;
 };
 
-// largest shapes over the instances of a batched launch (grid extents)
-struct BShape {
-    long Np = 0, Kp = 0;
-    int K = 0, D = 0, M1 = 0;
-};
+__host__ __device__ inline int bk_cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// workgroups of instance I in the flat grid of a kind
+__host__ __device__ inline int bk_count(const BInst &I, int kind) {
+    const auto cdiv = bk_cdiv;
+    switch (kind) {
+        case BK_MAKE_AK: return cdiv(I.Kp, 256);
+        case BK_REDUCE32: return cdiv(I.K, kBRK) * cdiv(I.D, kBRD);
+        case BK_ZERO_FIX: return I.D;
+        case BK_COPY_G: return cdiv(I.K, 256);
+        case BK_PART_AB: return cdiv(I.M1, kBGT) * cdiv(I.M1 + I.D, kBGT) * I.nchAB;
+        case BK_SUM_AB: return cdiv((long)I.M1 * I.M1 + (long)I.M1 * I.D, 256);
+        case BK_TRSM_LDS: return cdiv(I.D, 32);
+        case BK_PHIW: return cdiv(I.K, kBGT) * cdiv(I.D, kBGT);
+        case BK_ROWNORM: return cdiv(I.K, 256);
+        case BK_BETA_PART: return cdiv(I.K, kBBetaWG);
+        case BK_ABSMAX_PART: {
+            const long w = ((long)I.K * I.D + 4095) / 4096;
+            return (int)(w < kBAmaxWG ? w : kBAmaxWG);
+        }
+        case BK_MAKE_Y16: return cdiv(I.Kp, 4);
+        default: return 0;
+    }
+}
 
 // instances of one E-pass template class in a batched launch: their table
 // indices are list[first .. first + n), with items_e1 / items_e2 work items
@@ -132,6 +163,17 @@ struct BShape {
 struct BClass {
     int KS, TT, D, two, rlo;
     int first, n, items_e1, items_e2, index;
+    // flat grids of the chunk reduction and the zero-column fix over the
+    // class's instances: sizes and device maps
+    int wg_reduce32, wg_zero_fix;
+    const int2 *map_reduce32, *map_zero_fix;
+};
+
+// flat grids of the batched launches over their whole lists: sizes and
+// device maps (entry g = {table index, workgroup within that instance})
+struct BGrids {
+    int wg[BK_COUNT] = {};
+    const int2 *map[BK_COUNT] = {};
 };
 
 // batched E-step pieces (dg_mixed.hip)
@@ -142,18 +184,18 @@ int dg_batch_items(const BInst &I, int *e1, int *e2);
 // and the zero-column fix; st waits for every side stream at the end.
 // counters holds 4 ncls ints of device memory, ev 1 + nside events.
 int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
-                   int ncls, int *counters, const BShape &mx, int cus, hipStream_t *side,
+                   int ncls, int *counters, const BGrids &gr, int cus, hipStream_t *side,
                    int nside, hipEvent_t *ev);
 // the FP16 split of the new centres (dg_mixed_prepare_y) of the listed instances
 int dg_batch_prepare_y(hipStream_t st, const BInst *tab, const int *list, int n,
-                       const BShape &mx);
+                       const BGrids &gr);
 // out[i][kScalCount] = the scalars of table entry i, for i < n
 int dg_batch_gather_scal(hipStream_t st, const BInst *tab, int n, double *out);
 
 // M-step of the listed instances (dg_mstep.hip): list holds n table indices,
 // the nlds instances with M + 1 <= 126 (LDS Cholesky) first
 int dg_batch_mstep(hipStream_t st, const BInst *tab, const int *list, int n, int nlds,
-                   const BShape &mx);
+                   const BGrids &gr);
 
 // annealing of n uploaded mixed-precision contexts as batched launches per
 // step (dg_batch.cpp); graphs: capture each phase once as a hipGraph

@@ -1271,11 +1271,16 @@ __global__ void kb_zero_ints(int *__restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
+static_assert(RK == kBRK && RD == kBRD && kAmaxWG == kBAmaxWG, "flat-grid tile sizes");
+
+// Flat-grid kernels: workgroup blockIdx.x belongs to instance map[.].x and is
+// its workgroup map[.].y (grids sized by bk_count for each instance).
 __global__ __launch_bounds__(256) void kb_make_ak(const BInst *__restrict__ tab,
-                                                  const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_e || (int)blockIdx.x * 256 >= I.Kp) return;
-    k_make_ak_body(blockIdx.x, 0, 0, 0, I.yy32, I.K, I.Kp, I.beta_f, I.aK32);
+                                                  const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_e) return;
+    k_make_ak_body(m.y, 0, 0, 0, I.yy32, I.K, I.Kp, I.beta_f, I.aK32);
 }
 
 __global__ __launch_bounds__(1024) void kb_sum_partlog(const BInst *__restrict__ tab,
@@ -1286,45 +1291,45 @@ __global__ __launch_bounds__(1024) void kb_sum_partlog(const BInst *__restrict__
 }
 
 __global__ __launch_bounds__(RK * RD) void kb_reduce32(const BInst *__restrict__ tab,
-                                                       const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.z]];
-    if (!I.do_e || (int)blockIdx.x * RK >= I.K || (int)blockIdx.y * RD >= I.D) return;
-    k_reduce32_body(blockIdx.x, blockIdx.y, 0, 0, I.partRT32, I.partG32, I.NC, I.K, I.D, I.RT,
+                                                       const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_e) return;
+    const int nkx = (I.K + RK - 1) / RK;
+    k_reduce32_body(m.y % nkx, m.y / nkx, 0, 0, I.partRT32, I.partG32, I.NC, I.K, I.D, I.RT,
                     I.G);
 }
 
 __global__ __launch_bounds__(256) void kb_zero_fix(const BInst *__restrict__ tab,
-                                                   const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_e || (int)blockIdx.x >= I.D) return;
-    k_zero_fix_body(blockIdx.x, 0, 0, 0, I.scal, I.T64, I.ldt, I.pm, I.N, I.K, I.D, I.RT, I.G);
-}
-
-__device__ __forceinline__ int amax_wgs(const BInst &I) {
-    const long w = ((long)I.K * I.D + 4095) / 4096;
-    return (int)(w < kAmaxWG ? w : kAmaxWG);
+                                                   const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_e) return;
+    k_zero_fix_body(m.y, 0, 0, 0, I.scal, I.T64, I.ldt, I.pm, I.N, I.K, I.D, I.RT, I.G);
 }
 
 __global__ __launch_bounds__(256) void kb_absmax_part(const BInst *__restrict__ tab,
-                                                      const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    const int nwg = amax_wgs(I);
-    if (!I.do_m || (int)blockIdx.x >= nwg) return;
-    k_absmax_part_body(blockIdx.x, 0, 0, nwg, I.Y64, (long)I.K, I.ldt, I.D, I.amaxPart);
+                                                      const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_absmax_part_body(m.y, 0, 0, bk_count(I, BK_ABSMAX_PART), I.Y64, (long)I.K, I.ldt, I.D,
+                       I.amaxPart);
 }
 
 __global__ __launch_bounds__(256) void kb_absmax_final(const BInst *__restrict__ tab,
                                                        const int *__restrict__ list) {
     const BInst &I = tab[list[blockIdx.x]];
     if (!I.do_m) return;
-    k_absmax_final_body(0, 0, 0, 1, I.amaxPart, amax_wgs(I), I.scal + 6);
+    k_absmax_final_body(0, 0, 0, 1, I.amaxPart, bk_count(I, BK_ABSMAX_PART), I.scal + 6);
 }
 
 __global__ __launch_bounds__(256) void kb_make_y16(const BInst *__restrict__ tab,
-                                                   const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_m || (int)blockIdx.x * 4 >= I.Kp) return;
-    k_make_y16_body(blockIdx.x, 0, 0, 0, I.Y64, I.ldt, I.K, I.Kp, I.D, I.ldh, I.scal + 6, I.Y16,
+                                                   const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_make_y16_body(m.y, 0, 0, 0, I.Y64, I.ldt, I.K, I.Kp, I.D, I.ldh, I.scal + 6, I.Y16,
                     I.yy32, I.mscal);
 }
 
@@ -1607,13 +1612,13 @@ int dg_batch_items(const BInst &I, int *e1, int *e2) {
 }
 
 int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, const BClass *cls,
-                   int ncls, int *counters, const BShape &mx, int cus, hipStream_t *side,
+                   int ncls, int *counters, const BGrids &gr, int cus, hipStream_t *side,
                    int nside, hipEvent_t *ev) {
     if (n <= 0) return DG_OK;
     hipLaunchKernelGGL(kb_zero_ints, dim3((unsigned)((4 * ncls + 255) / 256)), dim3(256), 0, st,
                        counters, 4 * ncls);
-    hipLaunchKernelGGL(kb_make_ak, dim3((unsigned)((mx.Kp + 255) / 256), (unsigned)n), dim3(256), 0,
-                       st, tab, list);
+    hipLaunchKernelGGL(kb_make_ak, dim3((unsigned)gr.wg[BK_MAKE_AK]), dim3(256), 0, st, tab,
+                       gr.map[BK_MAKE_AK]);
     // the classes are independent until the scalars are gathered: class i runs
     // on side stream i % nside (fork after make_ak, join before the gather)
     if (hipEventRecord(ev[0], st) != hipSuccess) return DG_EHIP;
@@ -1632,10 +1637,10 @@ int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, con
         rc = (k.TT == 1) ? launch_class<1>(k, ss, tab, list, counters, true, cus)
                          : launch_class<2>(k, ss, tab, list, counters, true, cus);
         if (rc) return rc;
-        hipLaunchKernelGGL(kb_reduce32,
-                           dim3((unsigned)((mx.K + RK - 1) / RK), (unsigned)((mx.D + RD - 1) / RD), un),
-                           dim3(RK * RD), 0, ss, tab, l);
-        hipLaunchKernelGGL(kb_zero_fix, dim3((unsigned)mx.D, un), dim3(256), 0, ss, tab, l);
+        hipLaunchKernelGGL(kb_reduce32, dim3((unsigned)k.wg_reduce32), dim3(RK * RD), 0, ss, tab,
+                           k.map_reduce32);
+        hipLaunchKernelGGL(kb_zero_fix, dim3((unsigned)k.wg_zero_fix), dim3(256), 0, ss, tab,
+                           k.map_zero_fix);
     }
     for (int j = 0; j < nside; ++j) {
         if (hipEventRecord(ev[1 + j], side[j]) != hipSuccess) return DG_EHIP;
@@ -1645,13 +1650,13 @@ int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, con
 }
 
 int dg_batch_prepare_y(hipStream_t st, const BInst *tab, const int *list, int n,
-                       const BShape &mx) {
+                       const BGrids &gr) {
     if (n <= 0) return DG_OK;
-    const unsigned un = (unsigned)n;
-    hipLaunchKernelGGL(kb_absmax_part, dim3((unsigned)kAmaxWG, un), dim3(256), 0, st, tab, list);
-    hipLaunchKernelGGL(kb_absmax_final, dim3(un), dim3(256), 0, st, tab, list);
-    hipLaunchKernelGGL(kb_make_y16, dim3((unsigned)((mx.Kp + 3) / 4), un), dim3(256), 0, st, tab,
-                       list);
+    hipLaunchKernelGGL(kb_absmax_part, dim3((unsigned)gr.wg[BK_ABSMAX_PART]), dim3(256), 0, st, tab,
+                       gr.map[BK_ABSMAX_PART]);
+    hipLaunchKernelGGL(kb_absmax_final, dim3((unsigned)n), dim3(256), 0, st, tab, list);
+    hipLaunchKernelGGL(kb_make_y16, dim3((unsigned)gr.wg[BK_MAKE_Y16]), dim3(256), 0, st, tab,
+                       gr.map[BK_MAKE_Y16]);
     return hipGetLastError() == hipSuccess ? DG_OK : DG_EHIP;
 }
 

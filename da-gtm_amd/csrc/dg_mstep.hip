@@ -832,29 +832,34 @@ __global__ __launch_bounds__(256) void k_beta_final(const double *__restrict__ p
 // workgroups past them return before any barrier.  Every instance computes
 // exactly what its own single-instance launch computes.
 
+static_assert(GT == kBGT && kBetaWG == kBBetaWG, "flat-grid tile sizes");
+
+// Flat-grid kernels: workgroup blockIdx.x belongs to instance map[.].x and is
+// its workgroup map[.].y (grids sized by bk_count for each instance).
 __global__ __launch_bounds__(256) void kb_copy_g(const BInst *__restrict__ tab,
-                                                 const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    const int k = blockIdx.x * 256 + threadIdx.x;
+                                                 const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    const int k = m.y * 256 + threadIdx.x;
     if (I.do_m && k < I.K) I.Gm[k] = I.G[k];
 }
 
-constexpr int kBatchNchMax = 64;   // centre chunks of the A/B reduction, at most
-
 __global__ __launch_bounds__(256) void kb_partAB(const BInst *__restrict__ tab,
-                                                 const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.z / kBatchNchMax]];
-    const int c = blockIdx.z % kBatchNchMax;
-    if (!I.do_m || c >= I.nchAB || (int)blockIdx.x * GT >= I.M1 || (int)blockIdx.y * GT >= I.M1 + I.D)
-        return;
+                                                 const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    const int nx = (I.M1 + GT - 1) / GT, ny = (I.M1 + I.D + GT - 1) / GT;
     const int kc = (I.K + I.nchAB - 1) / I.nchAB;
-    k_partAB_body(blockIdx.x, blockIdx.y, c, 0, I.FI, I.G, I.RT, I.K, I.M1, I.D, kc, I.partAB);
+    k_partAB_body(m.y % nx, (m.y / nx) % ny, m.y / (nx * ny), 0, I.FI, I.G, I.RT, I.K, I.M1, I.D,
+                  kc, I.partAB);
 }
 
-__global__ void kb_sumAB(const BInst *__restrict__ tab, const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_m || (long)blockIdx.x * 256 >= (long)I.M1 * I.M1 + (long)I.M1 * I.D) return;
-    k_sumAB_body(blockIdx.x, 0, 0, 0, I.partAB, I.nchAB, I.M1, I.D, I.A0, I.Bm);
+__global__ void kb_sumAB(const BInst *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_sumAB_body(m.y, 0, 0, 0, I.partAB, I.nchAB, I.M1, I.D, I.A0, I.Bm);
 }
 
 __global__ __launch_bounds__(1024) void kb_chol_lds(const BInst *__restrict__ tab,
@@ -865,10 +870,11 @@ __global__ __launch_bounds__(1024) void kb_chol_lds(const BInst *__restrict__ ta
 }
 
 __global__ __launch_bounds__(1024) void kb_trsm_lds(const BInst *__restrict__ tab,
-                                                    const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_m || (int)blockIdx.x * 32 >= I.D) return;
-    k_trsm_lds_body(blockIdx.x, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
+                                                    const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_trsm_lds_body(m.y, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
 }
 
 __global__ __launch_bounds__(1024) void kb_chol_solve(const BInst *__restrict__ tab,
@@ -879,30 +885,34 @@ __global__ __launch_bounds__(1024) void kb_chol_solve(const BInst *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void kb_phiw(const BInst *__restrict__ tab,
-                                               const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.z]];
-    if (!I.do_m || (int)blockIdx.x * GT >= I.K || (int)blockIdx.y * GT >= I.D) return;
-    k_phiw_body(blockIdx.x, blockIdx.y, 0, 0, I.FI, I.W, I.K, I.M1, I.D, I.ldt, I.Y64);
+                                               const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    const int nx = (I.K + GT - 1) / GT;
+    k_phiw_body(m.y % nx, m.y / nx, 0, 0, I.FI, I.W, I.K, I.M1, I.D, I.ldt, I.Y64);
 }
 
-__global__ void kb_rownorm_y(const BInst *__restrict__ tab, const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_m || (long)blockIdx.x * 256 >= I.K) return;
-    k_rownorm_body(blockIdx.x, 0, 0, 0, I.Y64, I.K, I.ldt, I.D, I.yy);
+__global__ void kb_rownorm_y(const BInst *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_rownorm_body(m.y, 0, 0, 0, I.Y64, I.K, I.ldt, I.D, I.yy);
 }
 
 __global__ __launch_bounds__(256) void kb_beta_part(const BInst *__restrict__ tab,
-                                                    const int *__restrict__ list) {
-    const BInst &I = tab[list[blockIdx.y]];
-    if (!I.do_m || (int)blockIdx.x >= (I.K + kBetaWG - 1) / kBetaWG) return;
-    k_beta_part_body(blockIdx.x, 0, 0, 0, I.Y64, I.ldt, I.yy, I.G, I.RT, I.K, I.D, I.partAB);
+                                                    const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_m) return;
+    k_beta_part_body(m.y, 0, 0, 0, I.Y64, I.ldt, I.yy, I.G, I.RT, I.K, I.D, I.partAB);
 }
 
 __global__ __launch_bounds__(256) void kb_beta_final(const BInst *__restrict__ tab,
                                                      const int *__restrict__ list) {
     const BInst &I = tab[list[blockIdx.x]];
     if (!I.do_m) return;
-    k_beta_final_body(0, 0, 0, 1, I.partAB, (I.K + kBetaWG - 1) / kBetaWG, I.scal);
+    k_beta_final_body(0, 0, 0, 1, I.partAB, bk_count(I, BK_BETA_PART), I.scal);
 }
 
 }  // namespace
@@ -1041,36 +1051,30 @@ int dg_mstep_run(dg_ctx *c, double jitter) {
 }
 
 // M-step of the instances in list (n of them; nlds with M + 1 <= 126 first in
-// list, the others after), on one stream.  mx holds the largest K, D, M + 1
-// over them.  Each instance's steps are those of dg_mstep_run after the copy
-// of G into Gm.
+// list, the others after), on one stream; gr holds the flat grids over that
+// list (the LDS triangular solves over its first nlds entries).  Each
+// instance's steps are those of dg_mstep_run after the copy of G into Gm.
 int dg_batch_mstep(hipStream_t st, const BInst *tab, const int *list, int n, int nlds,
-                   const BShape &mx) {
+                   const BGrids &gr) {
     if (n <= 0) return DG_OK;
-    const unsigned un = (unsigned)n;
-    hipLaunchKernelGGL(kb_copy_g, dim3((unsigned)((mx.K + 255) / 256), un), dim3(256), 0, st, tab,
-                       list);
-    hipLaunchKernelGGL(kb_partAB,
-                       dim3((unsigned)((mx.M1 + GT - 1) / GT), (unsigned)((mx.M1 + mx.D + GT - 1) / GT),
-                            un * kBatchNchMax),
-                       dim3(256), 0, st, tab, list);
-    const long E = (long)mx.M1 * mx.M1 + (long)mx.M1 * mx.D;
-    hipLaunchKernelGGL(kb_sumAB, dim3((unsigned)((E + 255) / 256), un), dim3(256), 0, st, tab, list);
+    auto flat = [&](auto kernel, int kind, int threads) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)gr.wg[kind]), dim3(threads), 0, st, tab,
+                           gr.map[kind]);
+    };
+    flat(kb_copy_g, BK_COPY_G, 256);
+    flat(kb_partAB, BK_PART_AB, 256);
+    flat(kb_sumAB, BK_SUM_AB, 256);
     if (nlds > 0) {
         hipLaunchKernelGGL(kb_chol_lds, dim3((unsigned)nlds), dim3(1024), 0, st, tab, list);
-        hipLaunchKernelGGL(kb_trsm_lds, dim3((unsigned)((mx.D + 31) / 32), (unsigned)nlds),
-                           dim3(1024), 0, st, tab, list);
+        flat(kb_trsm_lds, BK_TRSM_LDS, 1024);
     }
     if (n > nlds)
         hipLaunchKernelGGL(kb_chol_solve, dim3((unsigned)(n - nlds)), dim3(1024), 0, st, tab,
                            list + nlds);
-    hipLaunchKernelGGL(kb_phiw, dim3((unsigned)((mx.K + GT - 1) / GT), (unsigned)((mx.D + GT - 1) / GT), un),
-                       dim3(256), 0, st, tab, list);
-    hipLaunchKernelGGL(kb_rownorm_y, dim3((unsigned)((mx.K + 255) / 256), un), dim3(256), 0, st, tab,
-                       list);
-    hipLaunchKernelGGL(kb_beta_part, dim3((unsigned)((mx.K + kBetaWG - 1) / kBetaWG), un), dim3(256),
-                       0, st, tab, list);
-    hipLaunchKernelGGL(kb_beta_final, dim3(un), dim3(256), 0, st, tab, list);
+    flat(kb_phiw, BK_PHIW, 256);
+    flat(kb_rownorm_y, BK_ROWNORM, 256);
+    flat(kb_beta_part, BK_BETA_PART, 256);
+    hipLaunchKernelGGL(kb_beta_final, dim3((unsigned)n), dim3(256), 0, st, tab, list);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? DG_OK : DG_EHIP;
 }
