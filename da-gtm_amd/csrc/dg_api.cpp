@@ -91,6 +91,7 @@ void free_problem(dg_ctx *c) {
     dfree(c->T64); dfree(c->tt); dfree(c->W); dfree(c->A); dfree(c->A0); dfree(c->Bm);
     dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
     dg_mixed_free(c);
+    dfree(c->perm);
     c->loaded = false;
     c->have_estep = false;
     c->have_w = false;
@@ -337,7 +338,9 @@ int dg_upload(dg_ctx *c, const double *T, long N, int D, long ldT, const double 
     c->loaded = true;
     if (c->mixed()) {
         c->nc_override = tune_find(c);
-        if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c))) return rc;
+        if ((rc = dg_mixed_prepare_t(c)) || (rc = dg_mixed_prepare_y(c)) ||
+            (rc = dg_mixed_order(c)))
+            return rc;
     }
     DG_HIP(c, hipStreamSynchronize(c->stream));
     return DG_OK;
@@ -901,6 +904,15 @@ int dg_device_peaks(dg_ctx *c, double *mfma_f16_tflops, double *exp_per_s, doubl
     if (exp_per_s) *exp_per_s = e;
     if (fp64_tflops) *fp64_tflops = f;
     return DG_OK;
+}
+
+int dg_pass2_blocks(dg_ctx *c, long *visited, long *total) {
+    if (!c || !visited || !total) return DG_EINVAL;
+    if (!c->mixed() || !c->have_estep)
+        return dg_fail(c, DG_ESTATE, "dg_pass2_blocks: no mixed-precision E-step yet");
+    int rc;
+    if ((rc = select_device(c))) return rc;
+    return dg_mixed_pass2_blocks(c, visited, total);
 }
 
 int dg_set_timing(dg_ctx *c, int on) {

@@ -237,10 +237,13 @@ int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *param
     for (BClass &k : b.cls) {
         const auto r1 = add_map(b.h_list + k.first, k.n, BK_REDUCE32);
         const auto r2 = add_map(b.h_list + k.first, k.n, BK_ZERO_FIX);
+        const auto r3 = add_map(b.h_list + k.first, k.n, BK_BLOCK_LISTS);
         k.wg_reduce32 = r1.second;
         k.wg_zero_fix = r2.second;
+        k.wg_lists = r3.second;
         at_cls.push_back(r1);
         at_cls.push_back(r2);
+        at_cls.push_back(r3);
     }
     for (int kind : {BK_COPY_G, BK_PART_AB, BK_SUM_AB, BK_PHIW, BK_ROWNORM, BK_BETA_PART})
         whole(b.h_list + n, n, kind);
@@ -250,8 +253,9 @@ int dg_batch_run_mixed(std::vector<dg_ctx *> &ctx, const dg_anneal_params *param
                               hipMemcpyHostToDevice));
     for (int kind = 0; kind < BK_COUNT; ++kind) b.gr.map[kind] = b.d_map + at[kind];
     for (size_t i = 0; i < b.cls.size(); ++i) {
-        b.cls[i].map_reduce32 = b.d_map + at_cls[2 * i].first;
-        b.cls[i].map_zero_fix = b.d_map + at_cls[2 * i + 1].first;
+        b.cls[i].map_reduce32 = b.d_map + at_cls[3 * i].first;
+        b.cls[i].map_zero_fix = b.d_map + at_cls[3 * i + 1].first;
+        b.cls[i].map_lists = b.d_map + at_cls[3 * i + 2].first;
     }
     DG_HIP(nullptr, hipMalloc((void **)&b.d_counters, sizeof(int) * 4 * b.cls.size()));
     DG_HIP(nullptr, hipMemcpy(b.d_list, b.h_list, sizeof(int) * 2 * n, hipMemcpyHostToDevice));

@@ -72,6 +72,8 @@ struct dg_ctx {
                                    // offsets then 32 underflow thresholds
     float *partRT32 = nullptr;     // NC x D x K
     float *partG32 = nullptr;      // NC x K
+    int *perm = nullptr;           // Np: input position of the mixed path's point n
+                                   // (points ordered by their best centre tile), or null
 
     int nc_override = 0;           // pass-2 chunk count chosen by the tuner (0: built-in)
     double *map_scratch = nullptr; // this device's dg_map_noalloc scratch (static device array)
@@ -102,7 +104,7 @@ enum {
 // per launch gives workgroup g its (table index, workgroup within instance).
 enum BKind {
     BK_MAKE_AK, BK_REDUCE32, BK_ZERO_FIX, BK_COPY_G, BK_PART_AB, BK_SUM_AB, BK_TRSM_LDS,
-    BK_PHIW, BK_ROWNORM, BK_BETA_PART, BK_ABSMAX_PART, BK_MAKE_Y16, BK_COUNT
+    BK_PHIW, BK_ROWNORM, BK_BETA_PART, BK_ABSMAX_PART, BK_MAKE_Y16, BK_BLOCK_LISTS, BK_COUNT
 };
 // tile sizes the flat grids are counted in (checked against the kernels)
 constexpr int kBRK = 64, kBRD = 16;     // k_reduce32: centres x dims per workgroup
@@ -122,7 +124,10 @@ struct BInst {
     // mixed state
     const _Float16 *T16, *T16t;
     _Float16 *Y16;
-    float *tt32, *yy32, *aK32, *pP32, *blkThr, *partRT32, *partG32;
+    float *tt32, *yy32, *aK32, *pP32, *blkThr, *partRT32, *partG32, *gmx, *bnd;
+    const int *perm;
+    int2 *brange;                  // pass-2 block range per (chunk, group)
+    float skipBelow;
     double *mscal, *partLog, *amaxPart;
     // step parameters
     float beta_f, it;
@@ -153,6 +158,7 @@ __host__ __device__ inline int bk_count(const BInst &I, int kind) {
             return (int)(w < kBAmaxWG ? w : kBAmaxWG);
         }
         case BK_MAKE_Y16: return cdiv(I.Kp, 4);
+        case BK_BLOCK_LISTS: return (I.Kp / 128) * I.NC;
         default: return 0;
     }
 }
@@ -165,8 +171,8 @@ struct BClass {
     int first, n, items_e1, items_e2, index;
     // flat grids of the chunk reduction and the zero-column fix over the
     // class's instances: sizes and device maps
-    int wg_reduce32, wg_zero_fix;
-    const int2 *map_reduce32, *map_zero_fix;
+    int wg_reduce32, wg_zero_fix, wg_lists;
+    const int2 *map_reduce32, *map_zero_fix, *map_lists;
 };
 
 // flat grids of the batched launches over their whole lists: sizes and
@@ -230,6 +236,12 @@ int dg_mixed_prepare_t(dg_ctx *ctx);
 long dg_mixed_nc_limits(const dg_ctx *ctx, long *lo, long *hi);
 int dg_mixed_nc(const dg_ctx *ctx);   // chunk count in use
 int dg_mixed_prepare_y(dg_ctx *ctx);
+// orders the mixed path's points by the centre tile holding their largest
+// responsibility under the current centres (one pass-1 sweep), then rebuilds
+// the FP16 operands in that order; DG_MIXED_ORDER=0 keeps the input order
+int dg_mixed_order(dg_ctx *ctx);
+// (centre group, block) pairs the last pass 2 visited, and all of them
+int dg_mixed_pass2_blocks(dg_ctx *ctx, long *visited, long *total);
 int dg_estep_mixed(dg_ctx *ctx, int isDA, double temp);
 void dg_mixed_free(dg_ctx *ctx);
 

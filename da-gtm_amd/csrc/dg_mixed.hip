@@ -24,6 +24,7 @@
 // straight into the second MFMA as an FP16 hi/lo pair without touching LDS.
 // Each chunk writes FP32 partials that a deterministic FP64 pass sums.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <cmath>
 
@@ -91,14 +92,18 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 // element would first need a v_accvgpr_read.
 // The body of one workgroup (point tile bx); k_e1_mfma runs one per
 // workgroup, kb_e1_mfma takes tiles of many instances from a work queue.
-template <int KS, int TT, bool TWO>
+// GMX: this instantiation keeps the centre-group maxima (gmx) and the pass-2
+// bounds (bnd); without it both pointers are ignored.
+template <int KS, int TT, bool TWO, bool GMX = true>
 __device__ __forceinline__ void e1_body(
     const long bx, const int tid, const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N,
     long Np, const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
-    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
+    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog,
+    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd) {
     static_assert(PT == 4 * BP, "one pass-1 wave covers one pass-2 block");
+    static_assert(CT == 4 * KT, "four pass-1 tiles make one pass-2 centre group");
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;                   // centre row in LDS (halves), padding last
     constexpr int RC = LR / 8;                     // 16-byte chunks per row
@@ -195,6 +200,8 @@ __device__ __forceinline__ void e1_body(
 
     float m = thr, S1 = 0.f, C1 = 0.f, S2 = 0.f, C2 = 0.f;
     float t1 = 0.f, t2 = 0.f, nmit = 0.f;
+    // the maximum over the current centre group (four tiles)
+    float gz = -INFINITY;
     // exponents of a tile from its accumulator, the tile maximum, the
     // underflow mask when it can matter, and the online rescaling
     auto tile_exp = [&](const f4v (&akr)[4], const f16v &c, float (&z)[16]) {
@@ -221,6 +228,7 @@ __device__ __forceinline__ void e1_body(
             }
             zt = xhalf_max(zt);
         }
+        if (GMX) gz = fmaxf(gz, zt);
         if (zt > m) {
             const float dm = m - zt;
             const float s1 = ex2(dm);
@@ -232,6 +240,16 @@ __device__ __forceinline__ void e1_body(
             m = zt;
         }
         nmit = -m * it;
+    };
+    // after the last tile of a centre group: its maximum per point (lanes of
+    // both halves hold the same value; the first half stores it)
+    // (a scalar base for this wave's 32 points plus the lane offset)
+    auto group_end = [&](const int tix) {
+        if (GMX && gmx && (tix & 3) == 3) {
+            float *gp = gmx + (long)(tix >> 2) * Np + bx * PT + 32 * wu;
+            if (h == 0) gp[j] = gz;
+            gz = -INFINITY;
+        }
     };
     // exponentials and sums of a tile, in steps: step 2r adds centre row r to
     // the first sum, step 2r+1 to the second (nothing when !TWO); row 0
@@ -317,6 +335,7 @@ __device__ __forceinline__ void e1_body(
         }
         tile_close();
         tile_exp(akr, c, z);
+        group_end(t + 1);
         __syncthreads();                           // also waits for this iteration's DMA
     };
     for (int t = 0; t + 1 < ntiles; t += 2) {
@@ -339,22 +358,24 @@ __device__ __forceinline__ void e1_body(
     if (h == 0) {
         const double s1 = ((double)S1 - C1) + ((double)S1o - C1o);
         const double s2 = TWO ? ((double)S2 - C2) + ((double)S2o - C2o) : s1;
+        // per-point results go to the point's input position
+        const long no = (perm && n < N) ? (long)perm[n] : n;
         if (n >= N) {
         } else if (!(s1 > 0.0)) {          // zero column (mpicomm.c:382-390)
             lg = isDA ? -INFINITY : logK;
             lh = logK;
             zc = 1.0;
-            pm[n] = -INFINITY;
-            logg[n] = lg;
-            logh[n] = lh;
+            pm[no] = -INFINITY;
+            logg[no] = lg;
+            logh[no] = lh;
         } else {
             const double L1 = log2(s1), L2 = log2(s2);
             const double ma = (double)m + (double)bn;        // the point's max exponent
             lg = kLn2 * (ma + L1);
             lh = kLn2 * (ma * it + L2);
-            pm[n] = ma;
-            logg[n] = lg;
-            logh[n] = lh;
+            pm[no] = ma;
+            logg[no] = lg;
+            logh[no] = lh;
             pbv = (float)(-(double)it * m - L2 + kRLog2);
             ptv = (float)((double)it * ((double)thr - m) - L2 + kRLog2);
         }
@@ -376,10 +397,40 @@ __device__ __forceinline__ void e1_body(
         zc += __shfl_xor(zc, o);
     }
     if (l == 0) { wsum[0][w] = lg; wsum[1][w] = lh; wsum[2][w] = zc; }
-    __syncthreads();
+    __syncthreads();                               // also orders the group maxima stored above
     if (tid < 3)
         partLog[bx * 3 + tid] =
             (wsum[tid][0] + wsum[tid][1]) + (wsum[tid][2] + wsum[tid][3]);
+    // Bounds for pass 2: for each centre group g, the largest pass-2 exponent
+    // it * z + pB that any (centre of g, point of this block) pair can reach,
+    // plus a rounding margin (pass 2 recomputes the exponents in another
+    // order).  Pass 2 skips a (group, block) pair whose bound shows every R
+    // there rounds to zero.  Lanes of half h take the groups g = h mod 2.
+    if (GMX && bnd) {
+        const float pb = h ? __shfl_xor(pbv, 32) : pbv;
+        const int ngr = Kp / CT;
+        const long blk = bx * (PT / BP) + wu, nblk = Np / BP;
+        const float *gp = gmx + bx * PT + 32 * wu + j;
+        constexpr int GB = 8;                      // groups in flight per half-wave
+        for (int g0 = 0; g0 < ngr; g0 += 2 * GB) {
+            float a[GB];
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                const int g = g0 + 2 * u + h;
+                a[u] = (g < ngr) ? gp[(long)g * Np] : -INFINITY;
+            }
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                const int g = g0 + 2 * u + h;
+                const float ai = it * a[u];
+                float v = ai + pb;
+                v = (v > -INFINITY) ? v + 1.0f + 1e-5f * (fabsf(ai) + fabsf(pb)) : -INFINITY;
+#pragma unroll
+                for (int o = 16; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+                if (j == 0 && g < ngr) bnd[(long)g * nblk + blk] = v;
+            }
+        }
+    }
 }
 
 template <int KS, int TT, bool TWO>
@@ -388,9 +439,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
     const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
-    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog) {
+    double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog,
+    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd) {
     e1_body<KS, TT, TWO>(blockIdx.x, threadIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal, beta, it, isDA, logK,
-                         pP, blkThr, pm, logg, logh, partLog);
+                         pP, blkThr, pm, logg, logh, partLog, perm, gmx, bnd);
 }
 
 // scal[0..2] = sums over the pass-1 workgroup partials (fixed tree order).
@@ -466,7 +518,7 @@ __device__ __forceinline__ void e2_body(
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
-    float *__restrict__ partRT, float *__restrict__ partG) {
+    float *__restrict__ partRT, float *__restrict__ partG, const int2 *__restrict__ brange) {
     constexpr int ld = 16 * KS;
     constexpr int LR = ld + PAD;                   // data row in LDS (halves), padding last
     constexpr int RC = LR / 8;                     // 16-byte chunks per data row
@@ -509,10 +561,21 @@ __device__ __forceinline__ void e2_body(
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
     float G = 0.f, Gc = 0.f;                       // VALU row sums when !GM
-    const long nb = cidx * chunk;
-    const long ne = (nb + chunk < Np) ? nb + chunk : Np;
-    const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
-    const _Float16 *T16tc = T16t + cidx * (long)(TT * DR) * chunk;   // this chunk's T^T
+    // The blocks this workgroup visits: the contiguous range of the chunk's
+    // blocks that k_block_ranges left (with the points ordered by their
+    // nearest centre, the blocks where some R of this centre group survives
+    // in FP16 are few and adjacent; outside the range every R there rounds to
+    // zero, so leaving those blocks out adds nothing).
+    // (read through a global address-space view and made wave-uniform, so
+    // that the batched kernel, whose pointers come from the instance table,
+    // keeps scalar DMA bases; the same for the mask thresholds below)
+    typedef const __attribute__((address_space(1))) int *gin;
+    typedef const __attribute__((address_space(1))) float *gfl;
+    const gin rgp = (gin)brange + 2 * (cidx * KG + kg);
+    const int rgx = __builtin_amdgcn_readfirstlane(rgp[0]);    // first block
+    const long nb = cidx * chunk + (long)rgx * BP;
+    const int nblocks = __builtin_amdgcn_readfirstlane(rgp[1]);
+    const _Float16 *T16tc = T16t + cidx * (long)(TT * DR) * chunk + (long)rgx * BP;  // T^T of the range
 
     // Staging: block q goes global -> LDS slot s by LDS-DMA.  The slot is a
     // sequence of 16-byte chunks [data rows | transposed rows | pB, pThr];
@@ -710,7 +773,8 @@ __device__ __forceinline__ void e2_body(
         __syncthreads();
         float e[16];
         cross_exp(0, e);
-        if (blkThr[blk0] > -kMaskMargin) mask(0, e);
+        const gfl thrg = (gfl)blkThr;
+        if (thrg[blk0] > -kMaskMargin) mask(0, e);
         // Steady state, one block per iteration, in a fixed issue order
         // (sched_barrier keeps the compiler from regrouping it):
         //   phase 1: cross-term MFMA m of block b+1, then step m of: block b's
@@ -797,7 +861,7 @@ __device__ __forceinline__ void e2_body(
             // index goes through readfirstlane so that the load stays a scalar
             // one: in the unrolled loop the compiler otherwise makes it a vector
             // load whose vmcnt(0) wait exposes a memory round trip per block.
-            if (blkThr[blk0 + __builtin_amdgcn_readfirstlane(b + 1)] > -kMaskMargin) mask(s1, en);
+            if (thrg[blk0 + __builtin_amdgcn_readfirstlane(b + 1)] > -kMaskMargin) mask(s1, en);
 #pragma unroll
             for (int r = 0; r < 16; ++r) e[r] = en[r];
             __syncthreads();                       // also waits for this iteration's DMA
@@ -846,9 +910,52 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
-    float *__restrict__ partRT, float *__restrict__ partG) {
+    float *__restrict__ partRT, float *__restrict__ partG, const int2 *__restrict__ brange) {
     e2_body<KS, TT, GM, RLO>(blockIdx.x, threadIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D, mscal, beta, it, pP,
-                             blkThr, chunk, KG, partRT, partG);
+                             blkThr, chunk, KG, partRT, partG, brange);
+}
+
+// The blocks pass 2 visits for one (point chunk, centre group) pair, lid =
+// chunk * KG + group: the smallest contiguous range holding every block whose
+// pass-1 bound is not below `below` (below it, every R of the group there
+// rounds to zero), or the whole chunk when bnd is null; brange[lid] = {first
+// block within the chunk, count}.  One workgroup per pair.
+__device__ __forceinline__ void block_range_body(const int lid, const float *__restrict__ bnd,
+                                                 long Np, long chunk, int KG, float below,
+                                                 int2 *__restrict__ brange) {
+    __shared__ int s_lo[NTH / 64], s_hi[NTH / 64];
+    const long cidx = lid / KG;
+    const int kg = lid % KG;
+    const long nb = cidx * chunk;
+    const long ne = (nb + chunk < Np) ? nb + chunk : Np;
+    const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
+    if (!bnd) {
+        if (threadIdx.x == 0) brange[lid] = make_int2(0, nblocks);
+        return;
+    }
+    const float *bb = bnd + (long)kg * (Np / BP) + nb / BP;
+    int lo = nblocks, hi = -1;
+    for (int q = threadIdx.x; q < nblocks; q += NTH)
+        if (!(bb[q] < below)) { lo = min(lo, q); hi = max(hi, q); }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_lo[w] = lo; s_hi[w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < NTH / 64; ++x) { lo = min(lo, s_lo[x]); hi = max(hi, s_hi[x]); }
+        lo = min(lo, s_lo[0]); hi = max(hi, s_hi[0]);
+        brange[lid] = (hi >= lo) ? make_int2(lo, hi - lo + 1) : make_int2(0, 0);
+    }
+}
+
+__global__ __launch_bounds__(NTH) void k_block_ranges(const float *__restrict__ bnd, long Np,
+                                                       long chunk, int KG, float below,
+                                                       int2 *__restrict__ brange) {
+    block_range_body(blockIdx.x, bnd, Np, chunk, KG, below, brange);
 }
 
 // ---------------------------------------------------------------- helpers
@@ -1001,14 +1108,16 @@ __device__ __forceinline__ double pow2_scale(double amax) {
 
 // Split T (N x D, stride ldt) into FP16 terms: T16[t][n][d], d < ld16, with
 // rows n >= N and columns d >= D zero; tt32[n] = |t'|^2 of the split value.
+// Row n holds input row perm[n] (row n when perm is null).
 __global__ void k_make_t16(const double *__restrict__ T64, int ldt, long N, long Np, int D,
-                           int ld16, int TT, double scale, _Float16 *__restrict__ T16,
-                           float *__restrict__ tt32) {
+                           int ld16, int TT, double scale, const int *__restrict__ perm,
+                           _Float16 *__restrict__ T16, float *__restrict__ tt32) {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= Np) return;
+    const long src = (n < N && perm) ? (long)perm[n] : n;      // input row of point n
     double ss = 0.0;
     for (int d = 0; d < ld16; ++d) {
-        const double v = (n < N && d < D) ? T64[n * ldt + d] * scale : 0.0;
+        const double v = (n < N && d < D) ? T64[src * ldt + d] * scale : 0.0;
         const _Float16 hi = (_Float16)(float)v;
         const _Float16 lo = (TT == 2) ? (_Float16)(float)(v - (double)(float)hi) : (_Float16)0.0f;
         T16[n * ld16 + d] = hi;
@@ -1089,6 +1198,50 @@ __global__ __launch_bounds__(256) void k_make_y16(const double *__restrict__ Y64
                     Y16, yy32, mscal);
 }
 
+// Nearest centre of each point from the first FP16 terms (an ordering key,
+// not a result): key[n] = argmax_k (t'_n . y'_k - s |y'_k|^2 / 2), s the
+// product of the two power-of-two operand scales (mscal[0] for Y, mscal[1]
+// for T), first k on ties.  One thread per point with its row in registers as
+// FP16 pairs (v_dot2_f32_f16); the centres pass through LDS 32 at a time.
+template <int KS>
+__global__ __launch_bounds__(256) void k_nearest(const _Float16 *__restrict__ T16, long N,
+                                                 const _Float16 *__restrict__ Y16,
+                                                 const float *__restrict__ yy32, int K,
+                                                 const double *__restrict__ mscal,
+                                                 int *__restrict__ key) {
+    constexpr int ld = 16 * KS, NPR = ld / 2, CTL = 32;
+    __shared__ h2 ys[CTL][NPR];
+    __shared__ float ysq[CTL];
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    h2 t[NPR];
+#pragma unroll
+    for (int p = 0; p < NPR; ++p) {
+        const h2 z = {(_Float16)0.0f, (_Float16)0.0f};
+        t[p] = (n < N) ? *(const h2 *)(T16 + n * ld + 2 * p) : z;
+    }
+    const float s = (float)(0.5 * mscal[0] * mscal[1]);
+    float best = -INFINITY;
+    int bk = 0;
+    for (int k0 = 0; k0 < K; k0 += CTL) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < CTL * NPR; e += 256) {
+            const int r = e / NPR, p = e % NPR;
+            const h2 z = {(_Float16)0.0f, (_Float16)0.0f};
+            ys[r][p] = (k0 + r < K) ? *(const h2 *)(Y16 + (long)(k0 + r) * ld + 2 * p) : z;
+        }
+        if (threadIdx.x < CTL) ysq[threadIdx.x] = (k0 + (int)threadIdx.x < K) ? s * yy32[k0 + threadIdx.x] : INFINITY;
+        __syncthreads();
+        for (int r = 0; r < CTL; ++r) {
+            float acc = 0.f;
+#pragma unroll
+            for (int p = 0; p < NPR; ++p) acc = __builtin_amdgcn_fdot2(t[p], ys[r][p], acc, false);
+            const float v = acc - ysq[r];
+            if (v > best) { best = v; bk = k0 + r; }
+        }
+    }
+    if (n < N) key[n] = bk;
+}
+
 // Supported numbers of 16-wide d steps (D is padded up to the next one).
 constexpr int kKS[] = {1, 2, 3, 4, 6, 8, 11, 12};
 
@@ -1107,10 +1260,26 @@ struct MixedBufs {
     float *blkThr = nullptr;      // per 32-point block: largest mask threshold
     float *aK32 = nullptr;        // Kp: pass-1 per-centre exponent term
     double *amaxPart = nullptr;   // 2 x kAmaxWG
+    float *gmx = nullptr;         // Kp/CT x Np: pass-1 maximum exponent per centre group and point
+    float *bnd = nullptr;         // Kp/CT x Np/BP: pass-2 exponent bound per group and block
+    int2 *brange = nullptr;       // NC x Kp/CT: pass-2 block range per (chunk, group)
     long Np = 0;
     long CL = 0;                  // points per pass-2 chunk (multiple of 64)
     int Kp = 0, KS = 0, TT = 1, NC = 1;
 };
+
+// Pass 2 skips a (centre group, block) pair whose exponent bound is below
+// this: every R there is below 2^-25 after the 2^14 carry, so both FP16
+// terms are zero; when G is summed from the FP32 values (no spare d row),
+// below 2^-160, so those are zero in FP32 as well.
+inline float skip_below(bool gm) { return gm ? -25.0f : -160.0f; }
+
+// DG_MIXED_SKIP=0 makes pass 2 visit every block; DG_MIXED_ORDER=0 keeps the
+// points in input order
+bool env_off(const char *name) {
+    const char *v = std::getenv(name);
+    return v && v[0] == '0' && v[1] == 0;
+}
 
 MixedBufs *mb(dg_ctx *c) { return reinterpret_cast<MixedBufs *>(c->mix); }
 
@@ -1120,12 +1289,12 @@ void launch_e2(dg_ctx *c, MixedBufs *b, float beta, float it, long chunk, int KG
         hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
                            0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
                            c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32);
+                           chunk, KG, c->partRT32, c->partG32, b->brange);
     else
         hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
                            0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
                            c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32);
+                           chunk, KG, c->partRT32, c->partG32, b->brange);
 }
 
 template <int KS, int TT>
@@ -1137,17 +1306,19 @@ void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int is
         hipLaunchKernelGGL((k_e1_mfma<KS, TT, true>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
                            b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
-                           c->pm, c->logg, c->logh, b->partLog);
+                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd);
     else
         hipLaunchKernelGGL((k_e1_mfma<KS, TT, false>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
                            b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
-                           c->pm, c->logg, c->logh, b->partLog);
+                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd);
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
     const long chunk = b->CL;
     const int KG = b->Kp / CT;
     // G comes out of the R T product when the padded d range has a spare row
     constexpr int DR = 32 * ((16 * KS + 31) / 32);
+    hipLaunchKernelGGL(k_block_ranges, dim3((unsigned)(KG * b->NC)), dim3(NTH), 0, c->stream,
+                       b->bnd, b->Np, chunk, KG, skip_below(DR > c->D), b->brange);
     if (DR > c->D) launch_e2<KS, TT, true>(c, b, beta, it, chunk, KG);
     else launch_e2<KS, TT, false>(c, b, beta, it, chunk, KG);
 }
@@ -1244,9 +1415,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         if (item >= total) break;
         const BInst &I = find_inst<false>(tab, list, nlist, item);
         if (!I.do_e) continue;
-        e1_body<KS, TT, TWO>(item - I.item_e1, opaque_tid(), I.T16, I.tt32, I.N, I.Np, I.Y16, I.aK32, I.Kp,
+        // no register room for the group maxima in the batched KS = 8, split-data
+        // pass 1 (those instances get no bounds, and pass 2 visits every block)
+        e1_body<KS, TT, TWO, !(KS == 8 && TT == 2)>(item - I.item_e1, opaque_tid(), I.T16, I.tt32, I.N, I.Np, I.Y16, I.aK32, I.Kp,
                              I.mscal, I.beta_f, I.it, I.isDA, I.logK, I.pP32, I.blkThr, I.pm,
-                             I.logg, I.logh, I.partLog);
+                             I.logg, I.logh, I.partLog, I.perm, I.gmx, I.bnd);
     }
 }
 
@@ -1262,7 +1435,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if (!I.do_e) continue;
         e2_body<KS, TT, GM, RLO>(item - I.item_e2, opaque_tid(), I.T16, I.T16t, I.Np, I.Y16, I.yy32, I.K, I.Kp,
                                  I.D, I.mscal, I.beta_f, I.it, I.pP32, I.blkThr, I.CL,
-                                 I.Kp / CT, I.partRT32, I.partG32);
+                                 I.Kp / CT, I.partRT32, I.partG32, I.brange);
     }
 }
 
@@ -1271,7 +1444,8 @@ __global__ void kb_zero_ints(int *__restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
-static_assert(RK == kBRK && RD == kBRD && kAmaxWG == kBAmaxWG, "flat-grid tile sizes");
+static_assert(RK == kBRK && RD == kBRD && kAmaxWG == kBAmaxWG && CT == 128,
+              "flat-grid tile sizes");
 
 // Flat-grid kernels: workgroup blockIdx.x belongs to instance map[.].x and is
 // its workgroup map[.].y (grids sized by bk_count for each instance).
@@ -1306,6 +1480,14 @@ __global__ __launch_bounds__(256) void kb_zero_fix(const BInst *__restrict__ tab
     const BInst &I = tab[m.x];
     if (!I.do_e) return;
     k_zero_fix_body(m.y, 0, 0, 0, I.scal, I.T64, I.ldt, I.pm, I.N, I.K, I.D, I.RT, I.G);
+}
+
+__global__ __launch_bounds__(NTH) void kb_block_ranges(const BInst *__restrict__ tab,
+                                                       const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    const BInst &I = tab[m.x];
+    if (!I.do_e) return;
+    block_range_body(m.y, I.bnd, I.Np, I.CL, I.Kp / CT, I.skipBelow, I.brange);
 }
 
 __global__ __launch_bounds__(256) void kb_absmax_part(const BInst *__restrict__ tab,
@@ -1433,6 +1615,9 @@ void dg_mixed_free(dg_ctx *c) {
         if (b->blkThr) (void)hipFree(b->blkThr);
         if (b->aK32) (void)hipFree(b->aK32);
         if (b->amaxPart) (void)hipFree(b->amaxPart);
+        if (b->gmx) (void)hipFree(b->gmx);
+        if (b->bnd) (void)hipFree(b->bnd);
+        if (b->brange) (void)hipFree(b->brange);
         delete b;
     }
     c->mix = nullptr;
@@ -1470,6 +1655,40 @@ long dg_mixed_nc_limits(const dg_ctx *c, long *lo, long *hi) {
 int dg_mixed_nc(const dg_ctx *c) {
     const MixedBufs *b = reinterpret_cast<const MixedBufs *>(c->mix);
     return b ? b->NC : 0;
+}
+
+int dg_mixed_pass2_blocks(dg_ctx *c, long *visited, long *total) {
+    MixedBufs *b = mb(c);
+    if (!b || !c->mixed_ready) return dg_fail(c, DG_ESTATE, "mixed buffers not prepared");
+    const int KG = b->Kp / CT;
+    std::vector<int2> rg((size_t)KG * b->NC);
+    DG_HIP(c, hipMemcpyAsync(rg.data(), b->brange, sizeof(int2) * rg.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    long v = 0;
+    for (const int2 &r : rg) v += r.y;
+    *visited = v;
+    *total = (long)KG * (b->Np / BP);
+    return DG_OK;
+}
+
+// T16 (both FP16 terms, rows in the mixed path's point order), tt32 and the
+// chunk-major transposed copy T16t
+int build_t16(dg_ctx *c, MixedBufs *b) {
+    const int ld16 = c->ldh;
+    const int DR = 32 * ((ld16 + 31) / 32);
+    hipLaunchKernelGGL(k_make_t16, dim3((unsigned)((b->Np + 255) / 256)), dim3(256), 0, c->stream,
+                       c->T64, c->ldt, c->N, b->Np, c->D, ld16, b->TT, (double)c->t_scale,
+                       c->perm, (_Float16 *)c->T16, c->tt32);
+    DG_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_transpose16,
+                       dim3((unsigned)((b->Np + 63) / 64), (unsigned)((DR + 63) / 64),
+                            (unsigned)b->TT),
+                       dim3(256), 0, c->stream, (const _Float16 *)c->T16, c->N, b->Np, ld16, DR,
+                       DR > c->D ? c->D : -1, b->CL, b->TT, b->T16t);
+    DG_HIP(c, hipGetLastError());
+    DG_HIP(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
 }
 
 int dg_mixed_prepare_t(dg_ctx *c) {
@@ -1532,21 +1751,71 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         (rc = dalloc(c, &c->partRT32, (size_t)b->NC * c->K * c->D)) ||
         (rc = dalloc(c, &c->partG32, (size_t)b->NC * c->K)))
         return rc;
+    // pass-2 block skipping: the group maxima and bounds pass 1 leaves
+    const size_t KG = (size_t)(b->Kp / CT);
+    if ((rc = dalloc(c, &b->brange, KG * b->NC))) return rc;
+    if (!env_off("DG_MIXED_SKIP") &&
+        ((rc = dalloc(c, &b->gmx, KG * b->Np)) || (rc = dalloc(c, &b->bnd, KG * (b->Np / BP)))))
+        return rc;
     double ms[4] = {1.0, tscale, 0.0, 0.0};
     DG_HIP(c, hipMemcpyAsync(b->mscal, ms, sizeof(ms), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_make_t16, dim3((unsigned)((b->Np + 255) / 256)), dim3(256), 0, c->stream,
-                       c->T64, c->ldt, c->N, b->Np, c->D, ld16, b->TT, tscale,
-                       (_Float16 *)c->T16, c->tt32);
-    DG_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_transpose16,
-                       dim3((unsigned)((b->Np + 63) / 64), (unsigned)((DR + 63) / 64),
-                            (unsigned)b->TT),
-                       dim3(256), 0, c->stream, (const _Float16 *)c->T16, c->N, b->Np, ld16, DR,
-                       DR > c->D ? c->D : -1, b->CL, b->TT, b->T16t);
-    DG_HIP(c, hipGetLastError());
-    DG_HIP(c, hipStreamSynchronize(c->stream));
+    int rcb = build_t16(c, b);
+    if (rcb) return rcb;
     c->mixed_ready = 1;
     return DG_OK;
+}
+
+// Point ordering: the points are ordered by their nearest centre under the
+// current centres (k_nearest; a stable counting sort, input order among
+// points with the same centre) and the FP16 operands rebuilt in that order.
+// Points with the same nearest centre have their responsibilities on the same
+// centres, so pass 2 finds whole (centre group, block) pairs with nothing to
+// add.  Any order gives the same results up to the order of the sums.
+int dg_mixed_order(dg_ctx *c) {
+    MixedBufs *b = mb(c);
+    if (!b || !c->mixed_ready) return dg_fail(c, DG_ESTATE, "mixed buffers not prepared");
+    if (c->perm || env_off("DG_MIXED_ORDER")) return DG_OK;
+    const long N = c->N, Np = b->Np;
+    const int K = c->K;
+    int *dkey = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &dkey, (size_t)N))) return rc;
+    const unsigned grid = (unsigned)((N + 255) / 256);
+    auto go = [&](auto ks) {
+        constexpr int KS = decltype(ks)::value;
+        hipLaunchKernelGGL((k_nearest<KS>), dim3(grid), dim3(256), 0, c->stream,
+                           (const _Float16 *)c->T16, N, b->Y16, c->yy32, K, b->mscal, dkey);
+    };
+    switch (b->KS) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 3: go(std::integral_constant<int, 3>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        case 6: go(std::integral_constant<int, 6>{}); break;
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        case 11: go(std::integral_constant<int, 11>{}); break;
+        default: go(std::integral_constant<int, 12>{}); break;
+    }
+    std::vector<int> key((size_t)N), perm((size_t)Np);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(key.data(), dkey, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dkey);
+    if (e != hipSuccess) return dg_hip_check(c, e, "point ordering");
+    std::vector<long> start((size_t)K + 1, 0);
+    for (long n = 0; n < N; ++n) {
+        const int k = key[(size_t)n];
+        if (k < 0 || k >= K) return dg_fail(c, DG_EHIP, "point ordering: bad centre index");
+        ++start[(size_t)k + 1];
+    }
+    for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+    for (long n = 0; n < N; ++n) perm[(size_t)start[(size_t)key[(size_t)n]]++] = (int)n;
+    for (long n = N; n < Np; ++n) perm[(size_t)n] = (int)n;
+    if ((rc = dalloc(c, &c->perm, (size_t)Np))) return rc;
+    DG_HIP(c, hipMemcpyAsync(c->perm, perm.data(), sizeof(int) * Np, hipMemcpyHostToDevice,
+                             c->stream));
+    return build_t16(c, b);
 }
 
 int dg_mixed_prepare_y(dg_ctx *c) {
@@ -1601,6 +1870,13 @@ int dg_batch_fill_mixed(const dg_ctx *c, BInst *I) {
     I->T16 = (const _Float16 *)c->T16; I->T16t = b->T16t; I->Y16 = b->Y16;
     I->tt32 = c->tt32; I->yy32 = c->yy32; I->aK32 = b->aK32; I->pP32 = c->pP32;
     I->blkThr = b->blkThr; I->partRT32 = c->partRT32; I->partG32 = c->partG32;
+    // the batched pass 1 keeps no group maxima for KS = 8 with split data
+    const bool gmx_ok = !(b->KS == 8 && b->TT == 2);
+    I->gmx = gmx_ok ? b->gmx : nullptr;
+    I->bnd = gmx_ok ? b->bnd : nullptr;
+    I->perm = c->perm;
+    I->brange = b->brange;
+    I->skipBelow = skip_below(32 * ((16 * b->KS + 31) / 32) > c->D);
     I->mscal = b->mscal; I->partLog = b->partLog; I->amaxPart = b->amaxPart;
     return DG_OK;
 }
@@ -1634,6 +1910,8 @@ int dg_batch_estep(hipStream_t st, const BInst *tab, const int *list, int n, con
                          : launch_class<2>(k, ss, tab, list, counters, false, cus);
         if (rc) return rc;
         hipLaunchKernelGGL(kb_sum_partlog, dim3(un), dim3(1024), 0, ss, tab, l);
+        hipLaunchKernelGGL(kb_block_ranges, dim3((unsigned)k.wg_lists), dim3(NTH), 0, ss, tab,
+                           k.map_lists);
         rc = (k.TT == 1) ? launch_class<1>(k, ss, tab, list, counters, true, cus)
                          : launch_class<2>(k, ss, tab, list, counters, true, cus);
         if (rc) return rc;

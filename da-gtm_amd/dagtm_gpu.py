@@ -22,7 +22,7 @@ EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_version", "dg_upload"
            "dg_set_state", "dg_set_basis", "dg_estep", "dg_mstep", "dg_anneal", "dg_next_ctemp", "dg_map",
            "dg_resp_dense", "dg_download", "dg_get_stats", "dg_set_timing",
            "dg_get_timing", "dg_batch_anneal", "dg_tune", "dg_anneal_ctl", "dg_map_noalloc",
-           "dg_device_peaks"]
+           "dg_device_peaks", "dg_pass2_blocks"]
 
 
 class DgConfig(C.Structure):
@@ -114,6 +114,7 @@ def load():
     L.dg_map_noalloc.argtypes = [vp, _dp, C.c_int, C.c_long, _dp, C.c_long]
     L.dg_device_peaks.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+    L.dg_pass2_blocks.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.dg_resp_dense.argtypes = [vp, _dp, C.c_long]
     L.dg_download.argtypes = [vp, vp, C.c_long, vp, C.c_long, C.POINTER(C.c_double)]
     L.dg_get_stats.argtypes = [vp, vp, vp, C.c_long]
@@ -331,6 +332,12 @@ class Engine:
         m, e, f = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.L.dg_device_peaks(self.h, C.byref(m), C.byref(e), C.byref(f)))
         return m.value, e.value, f.value
+
+    def pass2_blocks(self):
+        """(visited, total) (centre group, block) pairs of the last mixed E-pass 2."""
+        v, t = C.c_long(0), C.c_long(0)
+        self._chk(self.L.dg_pass2_blocks(self.h, C.byref(v), C.byref(t)))
+        return v.value, t.value
 
     def set_timing(self, on=True):
         self._chk(self.L.dg_set_timing(self.h, int(on)))

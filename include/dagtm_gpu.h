@@ -296,6 +296,13 @@ int dg_get_stats(dg_ctx *ctx, double *G, double *RT, long ldRT);
 int dg_device_peaks(dg_ctx *ctx, double *mfma_f16_tflops, double *exp_per_s,
                     double *fp64_tflops);
 
+/* Blocks of the last mixed-precision E-pass 2 (no counterpart in the
+ * reference; a measurement): *visited (centre group, 32-point block) pairs
+ * that it computed, of *total.  Pass 2 leaves out a pair only when every
+ * responsibility there rounds to zero in its arithmetic, so results do not
+ * depend on the count.  DG_ESTATE for an FP64 context or before an E-step. */
+int dg_pass2_blocks(dg_ctx *ctx, long *visited, long *total);
+
 /* Device time of the last dg_estep / dg_mstep, per kernel phase, in ms:
  * t[0] E-pass 1, t[1] E-pass 2, t[2] reductions, t[3] M-step.  Timing is
  * recorded only after dg_set_timing(ctx, 1). */
