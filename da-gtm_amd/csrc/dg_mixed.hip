@@ -506,15 +506,18 @@ __global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float b
 // 0-3, 8-11, 4-7, 12-15, so that one 16-byte LDS read gives a lane the eight
 // points its R operand holds (the accumulator layout of the cross term).
 // With GM, T16t row D is 1 for real points, and the R T product yields G in
-// that row.  The 1-D grid is mapped so that the workgroups the hardware places
-// on one XCD (linear id mod 8) share a point chunk and walk all its centre
-// groups, keeping that chunk's data in the XCD's L2.
-// The body of one workgroup (linear index L: a centre group and a point
-// chunk); k_e2_mfma runs one per workgroup, kb_e2_mfma takes them from a
-// work queue over many instances.
+// that row.  The work items, (point chunk, centre group) pairs, are mapped so
+// that the workgroups the hardware places on one XCD (linear id mod 8) take
+// the groups of one chunk after another, keeping that chunk's data in the
+// XCD's L2.  k_e2_mfma reads its item from a table (k_item_table) that lists,
+// per XCD, only the pairs with blocks to visit: the hardware hands out
+// workgroups in order, and pairs with nothing to do interleaved with long
+// ones would hold back the dispatch of the next workgroups.  kb_e2_mfma takes
+// its items from a work queue over many instances.  The body of one item:
+// centre group kg of point chunk cidx.
 template <int KS, int TT, bool GM, bool RLO>
 __device__ __forceinline__ void e2_body(
-    const int L, const int tid, const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
+    const long cidx, const int kg, const int tid, const _Float16 *__restrict__ T16, const _Float16 *__restrict__ T16t, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
@@ -539,9 +542,6 @@ __device__ __forceinline__ void e2_body(
     static_assert(NS * SLOT * 2 <= 160 * 1024, "LDS ring");
     __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
 
-    const int xs = L & 7, sl = L >> 3;
-    const int kg = sl % KG;
-    const long cidx = (long)(sl / KG) * 8 + xs;
     const int w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
     const int wu = __builtin_amdgcn_readfirstlane(w);
     const int k = kg * CT + 32 * w + j;
@@ -910,9 +910,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const _Float16 *__restrict__ Y16, const float *__restrict__ yy32, int K, int Kp, int D,
     const double *__restrict__ mscal, float beta, float it, const float *__restrict__ pP,
     const float *__restrict__ blkThr, long chunk, int KG,
-    float *__restrict__ partRT, float *__restrict__ partG, const int2 *__restrict__ brange) {
-    e2_body<KS, TT, GM, RLO>(blockIdx.x, threadIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D, mscal, beta, it, pP,
-                             blkThr, chunk, KG, partRT, partG, brange);
+    float *__restrict__ partRT, float *__restrict__ partG, const int2 *__restrict__ brange,
+    const int *__restrict__ itab, const int *__restrict__ icnt) {
+    const int x = blockIdx.x & 7, i = blockIdx.x >> 3;
+    if (i >= icnt[x]) return;                      // this XCD's list is shorter
+    const int lid = itab[blockIdx.x];
+    e2_body<KS, TT, GM, RLO>(lid / KG, lid % KG, threadIdx.x, T16, T16t, Np, Y16, yy32, K, Kp, D,
+                             mscal, beta, it, pP, blkThr, chunk, KG, partRT, partG, brange);
 }
 
 // The blocks pass 2 visits for one (point chunk, centre group) pair, lid =
@@ -958,6 +962,37 @@ __global__ __launch_bounds__(NTH) void k_block_ranges(const float *__restrict__ 
     block_range_body(blockIdx.x, bnd, Np, chunk, KG, below, brange);
 }
 
+// Pass 2's work list, one workgroup per XCD x: the (chunk, group) pairs of
+// chunks c = x, x + 8, ... (chunk after chunk, groups in order) whose block
+// range is not empty, at itab[x + 8 i] (i = 0, 1, ...), and their number
+// icnt[x].  Ballots and per-wave counts keep the order.
+__global__ __launch_bounds__(NTH) void k_item_table(const int2 *__restrict__ brange, int NC,
+                                                     int KG, int *__restrict__ itab,
+                                                     int *__restrict__ icnt) {
+    __shared__ int s_w[NTH / 64];
+    const int x = blockIdx.x;
+    const int nch = (NC - x + 7) / 8;              // this XCD's chunks
+    const int total = nch * KG;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int base = 0;
+    for (int q0 = 0; q0 < total; q0 += NTH) {
+        const int q = q0 + (int)threadIdx.x;
+        const int lid = (q / KG * 8 + x) * KG + q % KG;
+        const bool live = q < total && brange[lid].y > 0;
+        const unsigned long long bal = __ballot(live);
+        const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+        if (l == 0) s_w[w] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int y = 0; y < w; ++y) off += s_w[y];
+        if (live) itab[x + 8 * (off + pos)] = lid;
+        for (int y = 0; y < NTH / 64; ++y) base += s_w[y];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) icnt[x] = base;
+}
+
 // ---------------------------------------------------------------- helpers
 // RT[k][d] = sum_c partRT[c][d][k] and G[k] = sum_c partG[c][k], each in FP64
 // in chunk order.  One thread per output (lanes along the contiguous centre
@@ -970,12 +1005,16 @@ constexpr int RK = 64, RD = 16, RU = 8;
 __device__ __forceinline__ void k_reduce32_body(
     const int bx, const int by, const int bz, const int gx, const float *__restrict__ partRT,
     const float *__restrict__ partG, int NC, int K, int D, double *__restrict__ RT,
-    double *__restrict__ G) {
+    double *__restrict__ G, const int2 *__restrict__ brange) {
     __shared__ double tile[RK][RD + 1];
     const int t = threadIdx.x, kk = t & (RK - 1), dd = t / RK;
     const int k0 = bx * RK, d0 = by * RD;
     const int k = k0 + kk, d = d0 + dd;
     const long KD = (long)K * D;
+    // chunk c's partial sums of this centre group exist only where pass 2 had
+    // blocks to visit (brange[c][group].y > 0; no brange: everywhere)
+    const int KG = (K + CT - 1) / CT, grp = k0 / CT;
+    auto live = [&](int c) { return !brange || brange[(long)c * KG + grp].y > 0; };
     double s = 0.0;
     if (k < K && d < D) {
         const float *p = partRT + (long)d * K + k;
@@ -985,9 +1024,9 @@ __device__ __forceinline__ void k_reduce32_body(
 #pragma unroll
             for (int u = 0; u < RU; ++u) v[u] = p[(long)(c + u) * KD];
 #pragma unroll
-            for (int u = 0; u < RU; ++u) s += (double)v[u];
+            for (int u = 0; u < RU; ++u) s += live(c + u) ? (double)v[u] : 0.0;
         }
-        for (; c < NC; ++c) s += (double)p[(long)c * KD];
+        for (; c < NC; ++c) s += live(c) ? (double)p[(long)c * KD] : 0.0;
     }
     tile[kk][dd] = s;
     __syncthreads();
@@ -997,7 +1036,7 @@ __device__ __forceinline__ void k_reduce32_body(
     }
     if (by == 0 && t < RK && k < K) {
         double g = 0.0;
-        for (int c = 0; c < NC; ++c) g += (double)partG[(long)c * K + k];
+        for (int c = 0; c < NC; ++c) g += live(c) ? (double)partG[(long)c * K + k] : 0.0;
         G[k] = g;
     }
 }
@@ -1005,8 +1044,10 @@ __device__ __forceinline__ void k_reduce32_body(
 __global__ __launch_bounds__(RK * RD) void k_reduce32(const float *__restrict__ partRT,
                                                       const float *__restrict__ partG, int NC,
                                                       int K, int D, double *__restrict__ RT,
-                                                      double *__restrict__ G) {
-    k_reduce32_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, partRT, partG, NC, K, D, RT, G);
+                                                      double *__restrict__ G,
+                                                      const int2 *__restrict__ brange) {
+    k_reduce32_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, partRT, partG, NC, K, D, RT, G,
+                    brange);
 }
 
 // Zero-column fall-back R = 1/K for the points pass 2 skipped: RT[k] +=
@@ -1263,6 +1304,8 @@ struct MixedBufs {
     float *gmx = nullptr;         // Kp/CT x Np: pass-1 maximum exponent per centre group and point
     float *bnd = nullptr;         // Kp/CT x Np/BP: pass-2 exponent bound per group and block
     int2 *brange = nullptr;       // NC x Kp/CT: pass-2 block range per (chunk, group)
+    int *itab = nullptr;          // 8 x Kp/CT x ceil(NC/8): pass-2 work list per XCD
+    int *icnt = nullptr;          // 8: its lengths
     long Np = 0;
     long CL = 0;                  // points per pass-2 chunk (multiple of 64)
     int Kp = 0, KS = 0, TT = 1, NC = 1;
@@ -1285,16 +1328,17 @@ MixedBufs *mb(dg_ctx *c) { return reinterpret_cast<MixedBufs *>(c->mix); }
 
 template <int KS, int TT, bool GM>
 void launch_e2(dg_ctx *c, MixedBufs *b, float beta, float it, long chunk, int KG) {
+    const unsigned grid = (unsigned)(8 * KG * ((b->NC + 7) / 8));
     if (c->cfg.precision == DG_MIXED_R1)
-        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, false>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
-                           0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
-                           c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32, b->brange);
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, false>), dim3(grid), dim3(NTH), 0, c->stream,
+                           (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32, c->K, b->Kp,
+                           c->D, b->mscal, beta, it, c->pP32, b->blkThr, chunk, KG, c->partRT32,
+                           c->partG32, b->brange, b->itab, b->icnt);
     else
-        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, true>), dim3((unsigned)(KG * b->NC)), dim3(NTH),
-                           0, c->stream, (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16,
-                           c->yy32, c->K, b->Kp, c->D, b->mscal, beta, it, c->pP32, b->blkThr,
-                           chunk, KG, c->partRT32, c->partG32, b->brange);
+        hipLaunchKernelGGL((k_e2_mfma<KS, TT, GM, true>), dim3(grid), dim3(NTH), 0, c->stream,
+                           (const _Float16 *)c->T16, b->T16t, b->Np, b->Y16, c->yy32, c->K, b->Kp,
+                           c->D, b->mscal, beta, it, c->pP32, b->blkThr, chunk, KG, c->partRT32,
+                           c->partG32, b->brange, b->itab, b->icnt);
 }
 
 template <int KS, int TT>
@@ -1319,6 +1363,8 @@ void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int is
     constexpr int DR = 32 * ((16 * KS + 31) / 32);
     hipLaunchKernelGGL(k_block_ranges, dim3((unsigned)(KG * b->NC)), dim3(NTH), 0, c->stream,
                        b->bnd, b->Np, chunk, KG, skip_below(DR > c->D), b->brange);
+    hipLaunchKernelGGL(k_item_table, dim3(8), dim3(NTH), 0, c->stream, b->brange, b->NC, KG,
+                       b->itab, b->icnt);
     if (DR > c->D) launch_e2<KS, TT, true>(c, b, beta, it, chunk, KG);
     else launch_e2<KS, TT, false>(c, b, beta, it, chunk, KG);
 }
@@ -1433,7 +1479,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if (item >= total) break;
         const BInst &I = find_inst<true>(tab, list, nlist, item);
         if (!I.do_e) continue;
-        e2_body<KS, TT, GM, RLO>(item - I.item_e2, opaque_tid(), I.T16, I.T16t, I.Np, I.Y16, I.yy32, I.K, I.Kp,
+        // the instance's items in XCD-aware order, empty ones included (they
+        // write zero partial sums; the queue hands items out dynamically)
+        const int L = item - I.item_e2, KGi = I.Kp / CT, xs = L & 7, sl = L >> 3;
+        e2_body<KS, TT, GM, RLO>((long)(sl / KGi) * 8 + xs, sl % KGi, opaque_tid(), I.T16, I.T16t, I.Np, I.Y16, I.yy32, I.K, I.Kp,
                                  I.D, I.mscal, I.beta_f, I.it, I.pP32, I.blkThr, I.CL,
                                  I.Kp / CT, I.partRT32, I.partG32, I.brange);
     }
@@ -1471,7 +1520,7 @@ __global__ __launch_bounds__(RK * RD) void kb_reduce32(const BInst *__restrict__
     if (!I.do_e) return;
     const int nkx = (I.K + RK - 1) / RK;
     k_reduce32_body(m.y % nkx, m.y / nkx, 0, 0, I.partRT32, I.partG32, I.NC, I.K, I.D, I.RT,
-                    I.G);
+                    I.G, nullptr);
 }
 
 __global__ __launch_bounds__(256) void kb_zero_fix(const BInst *__restrict__ tab,
@@ -1618,6 +1667,8 @@ void dg_mixed_free(dg_ctx *c) {
         if (b->gmx) (void)hipFree(b->gmx);
         if (b->bnd) (void)hipFree(b->bnd);
         if (b->brange) (void)hipFree(b->brange);
+        if (b->itab) (void)hipFree(b->itab);
+        if (b->icnt) (void)hipFree(b->icnt);
         delete b;
     }
     c->mix = nullptr;
@@ -1753,7 +1804,9 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         return rc;
     // pass-2 block skipping: the group maxima and bounds pass 1 leaves
     const size_t KG = (size_t)(b->Kp / CT);
-    if ((rc = dalloc(c, &b->brange, KG * b->NC))) return rc;
+    if ((rc = dalloc(c, &b->brange, KG * b->NC)) ||
+        (rc = dalloc(c, &b->itab, 8 * KG * ((b->NC + 7) / 8))) || (rc = dalloc(c, &b->icnt, 8)))
+        return rc;
     if (!env_off("DG_MIXED_SKIP") &&
         ((rc = dalloc(c, &b->gmx, KG * b->Np)) || (rc = dalloc(c, &b->bnd, KG * (b->Np / BP)))))
         return rc;
@@ -1850,7 +1903,7 @@ int dg_estep_mixed(dg_ctx *c, int isDA, double temp) {
     hipLaunchKernelGGL(k_reduce32,
                        dim3((unsigned)((c->K + RK - 1) / RK), (unsigned)((c->D + RD - 1) / RD)),
                        dim3(RK * RD), 0, c->stream, c->partRT32, c->partG32, b->NC, c->K, c->D,
-                       c->RT, c->G);
+                       c->RT, c->G, b->brange);
     DG_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_sum_partlog, dim3(1), dim3(1024), 0, c->stream, b->partLog, b->Np / PT,
                        c->scal);
