@@ -126,8 +126,11 @@ __device__ __forceinline__ void e1_body(
     auto sp = [&](int s) -> _Float16 * { return s ? smem1 : smem0; };
     __shared__ double wsum[3][4];
 
-    const int w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
-    const int wu = __builtin_amdgcn_readfirstlane(w);
+    // (the & 3 bounds the wave index for the compiler when tid is opaque, as in
+    // the persistent kernels: it then still proves each LDS-DMA target disjoint
+    // from the LDS reads of the other ring slots and waits for no DMA there)
+    const int w = (tid >> 6) & 3, l = tid & 63, h = l >> 5, j = l & 31;
+    const int wu = __builtin_amdgcn_readfirstlane(w) & 3;
     const long n = bx * PT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
@@ -542,8 +545,11 @@ __device__ __forceinline__ void e2_body(
     static_assert(NS * SLOT * 2 <= 160 * 1024, "LDS ring");
     __shared__ __attribute__((aligned(1024))) _Float16 smem[NS][SLOT];
 
-    const int w = tid >> 6, l = tid & 63, h = l >> 5, j = l & 31;
-    const int wu = __builtin_amdgcn_readfirstlane(w);
+    // (the & 3 bounds the wave index for the compiler when tid is opaque, as in
+    // the persistent kernels: it then still proves each LDS-DMA target disjoint
+    // from the LDS reads of the other ring slots and waits for no DMA there)
+    const int w = (tid >> 6) & 3, l = tid & 63, h = l >> 5, j = l & 31;
+    const int wu = __builtin_amdgcn_readfirstlane(w) & 3;
     const int k = kg * CT + 32 * w + j;
     const float c1 = beta * kLog2e / (float)(mscal[0] * mscal[1]);
     const float cb = -0.5f * beta * kLog2e;
