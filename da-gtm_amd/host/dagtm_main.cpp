@@ -9,6 +9,7 @@
 #include <getopt.h>
 
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <csignal>
 #include <cstdio>
@@ -237,6 +238,13 @@ int main(int argc, char **argv) {
     int nloop = 0;
     int K = o.K;
     if (o.restart.empty()) {
+        auto t0 = std::chrono::steady_clock::now();
+        auto lap = [&t0]() {
+            const auto t = std::chrono::steady_clock::now();
+            const double s = std::chrono::duration<double>(t - t0).count();
+            t0 = t;
+            return s;
+        };
         std::vector<double> mu, sd;
         dgh::mean_sd(T, mu, sd);
         if (!dgh::setup_basis(o.L, o.K, o.M, o.s, X, FI)) {
@@ -245,19 +253,26 @@ int main(int argc, char **argv) {
         }
         if (o.seed == 0) o.seed = (unsigned long)std::time(nullptr);
         Mat Yrnd = dgh::random_centres(mu, sd, o.K, o.seed);
+        const double t_basis = lap();
         W = dgh::least_squares(FI, Yrnd);
+        if (W.rows == 0) {
+            std::fprintf(stderr, "least-squares W0: eigen-decomposition did not converge\n");
+            return 1;
+        }
+        Y = dgh::product(FI, W);
+        const double t_w0 = lap();
         beta = (o.K > 1000) ? 0.1 : dgh::initial_beta(Yrnd);
-        Y = Mat(K, D);
-        for (int k = 0; k < K; ++k)
-            for (int d = 0; d < D; ++d) {
-                double s = 0.0;
-                for (long i = 0; i < FI.cols; ++i) s += FI(k, i) * W(i, d);
-                Y(k, d) = s;
-            }
+        const double t_beta = lap();
         if (o.isDA) {
             if (o.need_ctemp) o.start_temp = dgh::first_critical_temp(T) / std::pow(o.alpha, 10);
             temp = o.start_temp;
         }
+        const double t_temp = lap();
+        if (o.verbose > 0)
+            std::fprintf(stderr,
+                         "init (%d host threads): statistics and basis %.3f s, W0 and Y0 %.3f s, "
+                         "beta0 %.3f s, first critical temperature %.3f s\n",
+                         dgh::pool().size(), t_basis, t_w0, t_beta, t_temp);
     } else {
         H5File ck = H5File::open_read(o.restart);
         int seed = 0;

@@ -2,14 +2,51 @@
 // seeded random start, least-squares W0, beta0 and the first critical
 // temperature.  Restates the reference's fresh-start chain
 // (dagtm.cxx:1104-1159; comm.c:992-1185, 1247-1294, 1401-1536;
-// mpicomm.c:511-559, 686-730) in FP64 on the CPU; it runs once per run.
+// mpicomm.c:511-559, 686-730) in FP64 on the CPU; it runs once per run, on
+// host_threads() workers.  Every output element is computed by one worker in
+// a fixed order, so the results do not depend on the number of threads.
 #pragma once
 
+#include <atomic>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace dgh {
+
+// Workers for the set-up chain: DG_HOST_THREADS when set, else the CPUs this
+// process may run on, at most 16.
+int host_threads();
+
+// A fixed set of worker threads; run(f) calls f(t) for every worker t in
+// [0, size()) (the calling thread is worker 0) and returns when all are done.
+class Pool {
+   public:
+    explicit Pool(int n);
+    ~Pool();
+    Pool(const Pool &) = delete;
+    Pool &operator=(const Pool &) = delete;
+    int size() const { return n_; }
+    void run(const std::function<void(int)> &f);
+
+   private:
+    void loop(int t);
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    const std::function<void(int)> *f_ = nullptr;
+    std::atomic<unsigned long> gen_{0};
+    std::atomic<int> left_{0};
+    std::atomic<bool> stop_{false};
+};
+
+// The process's pool of host_threads() workers, made on first use.
+Pool &pool();
 
 // Row-major dense matrix of doubles.
 struct Mat {
@@ -18,7 +55,7 @@ struct Mat {
     Mat() = default;
     Mat(long r, long c) : rows(r), cols(c), v((size_t)r * c, 0.0) {}
     double &operator()(long i, long j) { return v[(size_t)i * cols + j]; }
-    double operator()(long i, long j) const { return v[(size_t)i * cols + j]; }
+    const double &operator()(long i, long j) const { return v[(size_t)i * cols + j]; }
     double *data() { return v.data(); }
     const double *data() const { return v.data(); }
 };
@@ -49,13 +86,18 @@ Mat random_centres(const std::vector<double> &mu, const std::vector<double> &sd,
                    unsigned long seed);
 // Least-squares W minimising |FI W - Y| through the normal equations and a
 // symmetric eigen pseudo-inverse that drops exact-zero eigenvalues
-// (comm.c:1247-1294).
+// (comm.c:1247-1294).  Empty if the eigen-decomposition does not converge.
 Mat least_squares(const Mat &FI, const Mat &Y);
+// A B, each element summed over the inner index in order.
+Mat product(const Mat &A, const Mat &B);
 // 2 / mean nearest-neighbour squared distance (comm.c:1161-1185).
 double initial_beta(const Mat &Y);
 // First critical temperature lambda_max(Cov T) * beta_T (mpicomm.c:511-559).
 double first_critical_temp(const Mat &T);
-// Symmetric eigenvalues/vectors by cyclic Jacobi (A is overwritten).
-void eig_sym(int n, std::vector<double> &A, std::vector<double> &ev, std::vector<double> &V);
+// Eigenvalues ev (unordered) of the symmetric n x n row-major A (overwritten)
+// by Householder tridiagonalisation and implicit QL with Wilkinson shifts;
+// with Vt, eigenvector i is row i of Vt (n x n).  False if QL does not
+// converge.
+bool eig_sym(int n, std::vector<double> &A, std::vector<double> &ev, std::vector<double> *Vt);
 
 }  // namespace dgh
