@@ -709,15 +709,22 @@ static int batch_anneal_impl(const dg_config *cfg, int workers, const dg_batch_i
         });
     };
     using clk = std::chrono::steady_clock;
-    // phase 1: one context per instance, data resident on the device
+    // phase 1: one context per instance, data resident on the device, on the
+    // calling thread: every device allocation and free of the batch happens
+    // here or after phase 2, never while workers launch
     const auto t0 = clk::now();
-    run_pool([&](int i) {
+    for (int j = 0; j < n; ++j) {
+        const int i = order[j];
         const dg_batch_item &it = items[i];
-        int rc = dg_create(cfg, &ctx[i]);
-        if (!rc) rc = dg_upload(ctx[i], it.T, it.N, it.D, it.ldT, it.FI, it.K, it.M1, it.ldFI,
-                                it.Y0, it.ldY0, it.beta0);
-        status[i] = rc;
-    });
+        try {
+            int rc = dg_create(cfg, &ctx[i]);
+            if (!rc) rc = dg_upload(ctx[i], it.T, it.N, it.D, it.ldT, it.FI, it.K, it.M1, it.ldFI,
+                                    it.Y0, it.ldY0, it.beta0);
+            status[i] = rc;
+        } catch (...) {
+            status[i] = DG_ENOMEM;
+        }
+    }
     // phase 2: the annealing runs, concurrently on the workers' host threads,
     // or all instances' steps as batched launches
     const auto t1 = clk::now();
