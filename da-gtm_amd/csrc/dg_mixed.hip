@@ -410,7 +410,10 @@ __device__ __forceinline__ void e1_body(
     // order).  Pass 2 skips a (group, block) pair whose bound shows every R
     // there rounds to zero.  Lanes of half h take the groups g = h mod 2.
     if (GMX && bnd) {
-        const float pb = h ? __shfl_xor(pbv, 32) : pbv;
+        // the exchange runs on every lane: a lane reading a lane that is off
+        // (as under `h ? shuffle : own`) gets 0, not the value
+        const float pbo = __shfl_xor(pbv, 32);
+        const float pb = h ? pbo : pbv;
         const int ngr = Kp / CT;
         const long blk = bx * (PT / BP) + wu, nblk = Np / BP;
         const float *gp = gmx + bx * PT + 32 * wu + j;

@@ -15,8 +15,9 @@ are common.  Both are exact in the following sense, checked here:
   - several annealing steps along a trajectory where the skipped fraction
     grows stay bit-identical to the no-skip run.
 The workload: binary fingerprints (50 prototypes, flip 0.05) at D = 166 and
-K = 1728 (L = 3), and Gaussian-mixture data at D = 32 (no spare d row, so G
-comes from the FP32 sums and the stricter bound applies)."""
+K = 1728 (L = 3), the same with empty and two-bit points mixed in (points
+whose pass-2 offset is positive), and Gaussian-mixture data at D = 32 (no
+spare d row, so G comes from the FP32 sums and the stricter bound applies)."""
 import os
 import sys
 
@@ -35,8 +36,16 @@ def _say(msg):
 
 
 def _problem(kind):
-    if kind == "fp":
+    if kind in ("fp", "fp_sparse"):
         T = datagen.fingerprints(20000, 166, 7, nproto=50).astype(np.float64)
+        if kind == "fp_sparse":
+            # every 16th point empty or with two bits set: their largest
+            # exponent without the point's own norm term is negative, so their
+            # pass-2 offset pB is positive (a bound that dropped pB there would
+            # fall below the exponents it bounds)
+            rng = np.random.default_rng(3)
+            T[::16] = 0.0
+            T[8::32, rng.integers(0, 166, 2)] = 1.0
         X, FI = datagen.latent_basis(3, 1728, 64, 1.0)
     else:
         T = datagen.gmm(12000, 32, 5)
@@ -68,7 +77,7 @@ def _run(prob, env, steps, temp):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("kind", ["fp", "gmm"])
+@pytest.mark.parametrize("kind", ["fp", "gmm", "fp_sparse"])
 def test_skipping_is_bit_identical(kind):
     _say(f"case: pass-2 skipping, {kind}")
     prob = _problem(kind)
