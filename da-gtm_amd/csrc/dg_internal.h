@@ -126,7 +126,7 @@ struct BInst {
     _Float16 *Y16;
     float *tt32, *yy32, *aK32, *pP32, *blkThr, *partRT32, *partG32, *gmx, *bnd;
     const int *perm;
-    int2 *brange;                  // pass-2 block range per (chunk, group)
+    int2 *brange;                  // pass-2 block runs per (chunk, group)
     float skipBelow;
     double *mscal, *partLog, *amaxPart;
     // step parameters

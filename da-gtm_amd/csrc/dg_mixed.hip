@@ -45,6 +45,13 @@ constexpr int CT = 128;           // pass-2 centres per workgroup
 constexpr int KT = 32;            // pass-1 centres per tile
 constexpr int BP = 32;            // pass-2 points per block
 constexpr int PAD = 8;            // LDS row padding (halves)
+// Pass 2 visits, per (point chunk, centre group) pair, at most MAXR runs of
+// adjacent blocks; dead gaps of at most RGAP blocks stay inside a run.  The
+// pair's entry in the run table is RS int2: {runs, blocks}, then the runs
+// {first block within the chunk, count}.
+constexpr int MAXR = 15;
+constexpr int RS = MAXR + 1;
+constexpr int RGAP = 3;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kZmin = -1075.0f;  // log2 of the FP64 exp underflow point
 constexpr float kRLog2 = 14.0f;    // R is carried as R * 2^14 in FP16
@@ -570,21 +577,22 @@ __device__ __forceinline__ void e2_body(
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
     float G = 0.f, Gc = 0.f;                       // VALU row sums when !GM
-    // The blocks this workgroup visits: the contiguous range of the chunk's
-    // blocks that k_block_ranges left (with the points ordered by their
-    // nearest centre, the blocks where some R of this centre group survives
-    // in FP16 are few and adjacent; outside the range every R there rounds to
-    // zero, so leaving those blocks out adds nothing).
+    // The blocks this workgroup visits: the runs of the chunk's blocks that
+    // k_block_ranges listed (with the points ordered by their nearest centre,
+    // the blocks where some R of this centre group survives in FP16 are few
+    // and mostly adjacent; outside the runs every R there rounds to zero, so
+    // leaving those blocks out adds nothing).  The accumulators carry over
+    // from run to run; each run restarts the staging pipeline.
     // (read through a global address-space view and made wave-uniform, so
     // that the batched kernel, whose pointers come from the instance table,
     // keeps scalar DMA bases; the same for the mask thresholds below)
     typedef const __attribute__((address_space(1))) int *gin;
     typedef const __attribute__((address_space(1))) float *gfl;
-    const gin rgp = (gin)brange + 2 * (cidx * KG + kg);
-    const int rgx = __builtin_amdgcn_readfirstlane(rgp[0]);    // first block
-    const long nb = cidx * chunk + (long)rgx * BP;
-    const int nblocks = __builtin_amdgcn_readfirstlane(rgp[1]);
-    const _Float16 *T16tc = T16t + cidx * (long)(TT * DR) * chunk + (long)rgx * BP;  // T^T of the range
+    const gin hdr = (gin)brange + 2 * RS * (cidx * KG + kg);
+    const int nruns = __builtin_amdgcn_readfirstlane(hdr[0]);
+    int rgx = 0, nblocks = 0;                      // this run: first block, count
+    long nb = cidx * chunk;                        // its first point
+    const _Float16 *T16tc = T16t + cidx * (long)(TT * DR) * chunk;  // T^T of the run
 
     // Staging: block q goes global -> LDS slot s by LDS-DMA.  The slot is a
     // sequence of 16-byte chunks [data rows | transposed rows | pB, pThr];
@@ -610,16 +618,14 @@ __device__ __forceinline__ void e2_body(
         }
         voff[i] = o;
     }
-    // per DMA load of this wave: the source base at the chunk's first block
-    // and its step per block (both wave-uniform)
+    // per DMA load of this wave: the source base at the run's first block
+    // (set per run) and its step per block (both wave-uniform)
     const char *dbase[NIW];
     unsigned dstep[NIW];
 #pragma unroll
     for (int i = 0; i < NIW; ++i) {
         const int I = wu + 4 * i;
-        dbase[i] = (I < NIA) ? (const char *)(T16 + nb * ld)
-                 : (I < NIA + NIB) ? (const char *)T16tc
-                                   : (const char *)(pP + (nb / BP) * (2 * BP));
+        dbase[i] = nullptr;
         dstep[i] = (I < NIA) ? (unsigned)(BP * ld * 2) : (I < NIA + NIB) ? (unsigned)(BP * 2)
                                                                        : (unsigned)(2 * BP * 4);
     }
@@ -772,9 +778,24 @@ __device__ __forceinline__ void e2_body(
             rmfma(o, ldTt(s, 0, g), ldTt(s, TT - 1, g), rhi, rlo);
         }
     };
-    const long blk0 = nb / BP;                     // index of the chunk's first block
+    long blk0 = 0;                                 // index of the run's first block
+    const gfl thrg = (gfl)blkThr;
 
-    if (nblocks > 0) {
+    for (int run = 0; run < nruns; ++run) {
+        rgx = __builtin_amdgcn_readfirstlane(hdr[2 + 2 * run]);
+        nblocks = __builtin_amdgcn_readfirstlane(hdr[3 + 2 * run]);
+        nb = cidx * chunk + (long)rgx * BP;
+        T16tc = T16t + cidx * (long)(TT * DR) * chunk + (long)rgx * BP;
+        blk0 = nb / BP;
+#pragma unroll
+        for (int i = 0; i < NIW; ++i) {
+            const int I = wu + 4 * i;
+            dbase[i] = (I < NIA) ? (const char *)(T16 + nb * ld)
+                     : (I < NIA + NIB) ? (const char *)T16tc
+                                       : (const char *)(pP + blk0 * (2 * BP));
+        }
+        // the previous run's last reads of the ring come before this run's DMA
+        if (run > 0) __syncthreads();
 #pragma unroll
         for (int i = 0; i < NIW; ++i) dma(0, 0, i);
 #pragma unroll
@@ -782,7 +803,6 @@ __device__ __forceinline__ void e2_body(
         __syncthreads();
         float e[16];
         cross_exp(0, e);
-        const gfl thrg = (gfl)blkThr;
         if (thrg[blk0] > -kMaskMargin) mask(0, e);
         // Steady state, one block per iteration, in a fixed issue order
         // (sched_barrier keeps the compiler from regrouping it):
@@ -929,39 +949,65 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // The blocks pass 2 visits for one (point chunk, centre group) pair, lid =
-// chunk * KG + group: the smallest contiguous range holding every block whose
-// pass-1 bound is not below `below` (below it, every R of the group there
-// rounds to zero), or the whole chunk when bnd is null; brange[lid] = {first
-// block within the chunk, count}.  One workgroup per pair.
+// chunk * KG + group: the blocks whose pass-1 bound is not below `below`
+// (below it, every R of the group there rounds to zero) as runs of adjacent
+// blocks, dead gaps of up to RGAP blocks merged into a run and the last run
+// taking in everything after it once MAXR runs are used; the whole chunk as
+// one run when bnd is null.  brange[RS lid] = {runs, blocks}, then the runs
+// {first block within the chunk, count}.  One workgroup per pair: the waves
+// ballot a segment of NTH blocks, the first lane walks the live ones.
 __device__ __forceinline__ void block_range_body(const int lid, const float *__restrict__ bnd,
                                                  long Np, long chunk, int KG, float below,
                                                  int2 *__restrict__ brange) {
-    __shared__ int s_lo[NTH / 64], s_hi[NTH / 64];
+    __shared__ unsigned long long s_live[NTH / 64];
     const long cidx = lid / KG;
     const int kg = lid % KG;
     const long nb = cidx * chunk;
     const long ne = (nb + chunk < Np) ? nb + chunk : Np;
     const int nblocks = (ne > nb) ? (int)((ne - nb) / BP) : 0;
+    int2 *out = brange + (long)RS * lid;
     if (!bnd) {
-        if (threadIdx.x == 0) brange[lid] = make_int2(0, nblocks);
+        if (threadIdx.x == 0) {
+            out[0] = make_int2(nblocks > 0 ? 1 : 0, nblocks);
+            out[1] = make_int2(0, nblocks);
+        }
         return;
     }
     const float *bb = bnd + (long)kg * (Np / BP) + nb / BP;
-    int lo = nblocks, hi = -1;
-    for (int q = threadIdx.x; q < nblocks; q += NTH)
-        if (!(bb[q] < below)) { lo = min(lo, q); hi = max(hi, q); }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, __shfl_xor(lo, o));
-        hi = max(hi, __shfl_xor(hi, o));
-    }
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { s_lo[w] = lo; s_hi[w] = hi; }
-    __syncthreads();
+    int nr = 0, cs = -1, cl = -1, tot = 0;         // first lane: runs, current run, blocks
+    for (int q0 = 0; q0 < nblocks; q0 += NTH) {
+        const int q = q0 + (int)threadIdx.x;
+        const unsigned long long bal = __ballot(q < nblocks && !(bb[q] < below));
+        if ((threadIdx.x & 63) == 0) s_live[w] = bal;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int x = 0; x < NTH / 64; ++x) {
+                unsigned long long m = s_live[x];
+                while (m) {
+                    const int b = q0 + 64 * x + __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (cs < 0) {
+                        cs = b;
+                    } else if (b - cl - 1 > RGAP && nr < MAXR - 1) {
+                        out[1 + nr] = make_int2(cs, cl - cs + 1);
+                        tot += cl - cs + 1;
+                        ++nr;
+                        cs = b;
+                    }
+                    cl = b;
+                }
+            }
+        }
+        __syncthreads();                           // s_live is rewritten next segment
+    }
     if (threadIdx.x == 0) {
-        for (int x = 1; x < NTH / 64; ++x) { lo = min(lo, s_lo[x]); hi = max(hi, s_hi[x]); }
-        lo = min(lo, s_lo[0]); hi = max(hi, s_hi[0]);
-        brange[lid] = (hi >= lo) ? make_int2(lo, hi - lo + 1) : make_int2(0, 0);
+        if (cs >= 0) {
+            out[1 + nr] = make_int2(cs, cl - cs + 1);
+            tot += cl - cs + 1;
+            ++nr;
+        }
+        out[0] = make_int2(nr, tot);
     }
 }
 
@@ -987,7 +1033,7 @@ __global__ __launch_bounds__(NTH) void k_item_table(const int2 *__restrict__ bra
     for (int q0 = 0; q0 < total; q0 += NTH) {
         const int q = q0 + (int)threadIdx.x;
         const int lid = (q / KG * 8 + x) * KG + q % KG;
-        const bool live = q < total && brange[lid].y > 0;
+        const bool live = q < total && brange[(long)RS * lid].y > 0;
         const unsigned long long bal = __ballot(live);
         const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
@@ -1021,9 +1067,10 @@ __device__ __forceinline__ void k_reduce32_body(
     const int k = k0 + kk, d = d0 + dd;
     const long KD = (long)K * D;
     // chunk c's partial sums of this centre group exist only where pass 2 had
-    // blocks to visit (brange[c][group].y > 0; no brange: everywhere)
+    // blocks to visit (blocks > 0 in the pair's run-table entry; no table:
+    // everywhere)
     const int KG = (K + CT - 1) / CT, grp = k0 / CT;
-    auto live = [&](int c) { return !brange || brange[(long)c * KG + grp].y > 0; };
+    auto live = [&](int c) { return !brange || brange[RS * ((long)c * KG + grp)].y > 0; };
     double s = 0.0;
     if (k < K && d < D) {
         const float *p = partRT + (long)d * K + k;
@@ -1312,7 +1359,7 @@ struct MixedBufs {
     double *amaxPart = nullptr;   // 2 x kAmaxWG
     float *gmx = nullptr;         // Kp/CT x Np: pass-1 maximum exponent per centre group and point
     float *bnd = nullptr;         // Kp/CT x Np/BP: pass-2 exponent bound per group and block
-    int2 *brange = nullptr;       // NC x Kp/CT: pass-2 block range per (chunk, group)
+    int2 *brange = nullptr;       // NC x Kp/CT x RS: pass-2 block runs per (chunk, group)
     int *itab = nullptr;          // 8 x Kp/CT x ceil(NC/8): pass-2 work list per XCD
     int *icnt = nullptr;          // 8: its lengths
     long Np = 0;
@@ -1721,12 +1768,12 @@ int dg_mixed_pass2_blocks(dg_ctx *c, long *visited, long *total) {
     MixedBufs *b = mb(c);
     if (!b || !c->mixed_ready) return dg_fail(c, DG_ESTATE, "mixed buffers not prepared");
     const int KG = b->Kp / CT;
-    std::vector<int2> rg((size_t)KG * b->NC);
+    std::vector<int2> rg((size_t)KG * b->NC * RS);
     DG_HIP(c, hipMemcpyAsync(rg.data(), b->brange, sizeof(int2) * rg.size(), hipMemcpyDeviceToHost,
                              c->stream));
     DG_HIP(c, hipStreamSynchronize(c->stream));
     long v = 0;
-    for (const int2 &r : rg) v += r.y;
+    for (size_t i = 0; i < rg.size(); i += RS) v += rg[i].y;
     *visited = v;
     *total = (long)KG * (b->Np / BP);
     return DG_OK;
@@ -1813,7 +1860,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         return rc;
     // pass-2 block skipping: the group maxima and bounds pass 1 leaves
     const size_t KG = (size_t)(b->Kp / CT);
-    if ((rc = dalloc(c, &b->brange, KG * b->NC)) ||
+    if ((rc = dalloc(c, &b->brange, (size_t)KG * b->NC * RS)) ||
         (rc = dalloc(c, &b->itab, 8 * KG * ((b->NC + 7) / 8))) || (rc = dalloc(c, &b->icnt, 8)))
         return rc;
     if (!env_off("DG_MIXED_SKIP") &&
