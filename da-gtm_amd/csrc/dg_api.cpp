@@ -92,6 +92,7 @@ void free_problem(dg_ctx *c) {
     dfree(c->pm); dfree(c->pl); dfree(c->logg); dfree(c->logh);
     dg_mixed_free(c);
     dfree(c->perm);
+    dfree(c->p1start);
     c->loaded = false;
     c->have_estep = false;
     c->have_w = false;

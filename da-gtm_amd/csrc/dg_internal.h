@@ -73,6 +73,7 @@ struct dg_ctx {
     float *partRT32 = nullptr;     // NC x D x K
     float *partG32 = nullptr;      // NC x K
     int *perm = nullptr;           // Np: input position of the mixed path's point n
+    int *p1start = nullptr;        // Np/128: the tile each pass-1 workgroup starts from
                                    // (points ordered by their best centre tile), or null
 
     int nc_override = 0;           // pass-2 chunk count chosen by the tuner (0: built-in)
@@ -126,6 +127,8 @@ struct BInst {
     _Float16 *Y16;
     float *tt32, *yy32, *aK32, *pP32, *blkThr, *partRT32, *partG32, *gmx, *bnd;
     const int *perm;
+    const int *p1start;            // pass-1 start tile per point tile (null: tile 0)
+    int screen;                    // pass 1 screens tiles
     int2 *brange;                  // pass-2 block runs per (chunk, group)
     float skipBelow;
     double *mscal, *partLog, *amaxPart;

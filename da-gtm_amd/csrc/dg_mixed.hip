@@ -44,6 +44,7 @@ constexpr int PT = 128;           // pass-1 points per workgroup
 constexpr int CT = 128;           // pass-2 centres per workgroup
 constexpr int KT = 32;            // pass-1 centres per tile
 constexpr int BP = 32;            // pass-2 points per block
+constexpr int AKS = KT + 4;       // pass-1 aK per tile: 32 values, the largest, 3 pads
 constexpr int PAD = 8;            // LDS row padding (halves)
 // Pass 2 visits, per (point chunk, centre group) pair, at most MAXR runs of
 // adjacent blocks; dead gaps of at most RGAP blocks stay inside a run.  The
@@ -101,14 +102,23 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
 // workgroup, kb_e1_mfma takes tiles of many instances from a work queue.
 // GMX: this instantiation keeps the centre-group maxima (gmx) and the pass-2
 // bounds (bnd); without it both pointers are ignored.
-template <int KS, int TT, bool TWO, bool GMX = true>
+// SCR (data exact in FP16, GMX): a tile's cross term takes the hi term of Y
+// first; when that product plus a bound on the lo term puts every element of
+// the wave's tile more than `smargin` below its point's running maximum, the
+// lo term and the tile's exponentials are left out (each such element weighs
+// less than 2^-48 of its point's sums) and the bound enters the group maximum.
+// t0: the tile the sweep starts from (a multiple of four, so that groups stay
+// aligned); the sweep wraps around.
+template <int KS, int TT, bool TWO, bool GMX = true, bool SCR = false>
 __device__ __forceinline__ void e1_body(
     const long bx, const int tid, const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N,
     long Np, const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
     double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog,
-    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd) {
+    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd,
+    const int t0 = 0, const float smargin = INFINITY) {
+    static_assert(!SCR || (TT == 1 && GMX), "screening needs exact data and the group maxima");
     static_assert(PT == 4 * BP, "one pass-1 wave covers one pass-2 block");
     static_assert(CT == 4 * KT, "four pass-1 tiles make one pass-2 centre group");
     constexpr int ld = 16 * KS;
@@ -153,6 +163,19 @@ __device__ __forceinline__ void e1_body(
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
             tb[t][ks] = *(const h8 *)(T16 + ((long)t * Np + n) * ld + 16 * ks + 8 * h);
+    // SCR: the lo term's share of an exponent is at most c1 * 8 * sum_d |t'_d|
+    // (the split leaves |ylo'| <= 8 for values below 2^14), plus one unit of
+    // slack for the FP32 rounding of the hi-term product
+    float errz = 0.f;
+    if (SCR) {
+        float a = 0.f;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) a += fabsf((float)tb[0][ks][e]);
+        a += __shfl_xor(a, 32);                    // the other half of d
+        errz = c1 * 8.0f * a * 1.001f + 1.0f;
+    }
 
     // DMA: load I of a tile fills chunks 64 I .. 64 I + 63 of the slot in lane
     // order ([Y hi rows | Y lo rows | aK]); wave w issues loads w, w + 4, ...
@@ -169,15 +192,17 @@ __device__ __forceinline__ void e1_body(
             const int t = g / (KT * RC), rem = g % (KT * RC), row = rem / RC, col = rem % RC;
             if (t < 2 && col < ld / 8) o = (unsigned)((((long)t * Kp + row) * ld + col * 8) * 2);
         } else if (I == NIA) {
-            if (l < KT / 4) o = (unsigned)(l * 16);
+            if (l < AKS / 4) o = (unsigned)(l * 16);   // the tile's aK, its largest, pads
         }
         voff[i] = o;
         dbase[i] = (I < NIA) ? (const char *)Y16 : (const char *)aK32;
-        dstep[i] = (I < NIA) ? (unsigned)(KT * ld * 2) : (unsigned)(KT * 4);
+        dstep[i] = (I < NIA) ? (unsigned)(KT * ld * 2) : (unsigned)(AKS * 4);
     }
     const int ntiles = Kp / KT;
+    // loop position q holds tile q + t0 (mod ntiles)
+    auto tile_of = [&](int q) { return (q + t0 < ntiles) ? q + t0 : q + t0 - ntiles; };
     auto dma = [&](int q, int s, int i) {
-        const unsigned qc = (unsigned)(q < ntiles ? q : ntiles - 1);
+        const unsigned qc = (unsigned)tile_of(q < ntiles ? q : ntiles - 1);
         const int I = wu + 4 * i;
         __builtin_amdgcn_global_load_lds(
             (const void *)(dbase[i] + qc * dstep[i] + voff[i]),
@@ -199,10 +224,17 @@ __device__ __forceinline__ void e1_body(
     auto ldK = [&](int s, int m4) {                // aK of rows 8 m4 + 4 h .. + 3
         return *(const f4v *)&sp(s)[OFF_AK + 2 * (8 * m4 + 4 * h)];
     };
+    auto ldKmax = [&](int s) {                     // the tile's largest aK
+        return *(const float *)&sp(s)[OFF_AK + 2 * KT];
+    };
     // cross-term MFMA number m (m = (TT + 1) ks + which) of the tile whose
     // Y rows are a0 (hi) / a1 (lo)
+    // (with the screen, every hi-term MFMA comes first and the lo terms after
+    // them: pass 2 accumulates its cross term in the same order, so both
+    // passes form bit-identical exponents)
+    auto xks = [&](int m) { return SCR ? (m < KS ? m : m - KS) : m / (TT + 1); };
     auto xmfma = [&](int m, const h8 &a0, const h8 &a1, f16v &c) {
-        const int ks = m / (TT + 1), wq = m % (TT + 1);
+        const int ks = xks(m), wq = SCR ? (m < KS ? 0 : 1) : m % (TT + 1);
         if (wq == 0) c = mfma(a0, tb[0][ks], c);
         else if (wq == 1) c = mfma(a1, tb[0][ks], c);
         else c = mfma(a0, tb[TT - 1][ks], c);
@@ -256,7 +288,7 @@ __device__ __forceinline__ void e1_body(
     // (a scalar base for this wave's 32 points plus the lane offset)
     auto group_end = [&](const int tix) {
         if (GMX && gmx && (tix & 3) == 3) {
-            float *gp = gmx + (long)(tix >> 2) * Np + bx * PT + 32 * wu;
+            float *gp = gmx + (long)(tile_of(tix) >> 2) * Np + bx * PT + 32 * wu;
             if (h == 0) gp[j] = gz;
             gz = -INFINITY;
         }
@@ -296,7 +328,7 @@ __device__ __forceinline__ void e1_body(
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
 #pragma unroll
         for (int mm = 0; mm < NX; ++mm) {
-            const int ks = mm / (TT + 1);
+            const int ks = xks(mm);
             xmfma(mm, ldY(0, 0, ks), ldY(0, 1, ks), c);
         }
         f4v ak0[4];
@@ -348,14 +380,101 @@ __device__ __forceinline__ void e1_body(
         group_end(t + 1);
         __syncthreads();                           // also waits for this iteration's DMA
     };
-    for (int t = 0; t + 1 < ntiles; t += 2) {
-        iter1(t, std::integral_constant<int, 1>{});
-        if (t + 2 >= ntiles) break;
-        iter1(t + 1, std::integral_constant<int, 0>{});
-    }
+    // SCR iteration t: phase A, tile t+1's hi-term MFMAs with tile t's aK
+    // read, the DMA loads of tile t+2 and the first RA row sums of tile t;
+    // the screen; phase B, the lo-term MFMAs with the other row sums (or the
+    // row sums alone when tile t+1 is screened out).  skip_t: tile t was
+    // screened out (no row sums); returns whether tile t+1 is.
+    constexpr int RA = (NRW - NIW) / 2 > 0 ? (NRW - NIW) / 2 : 0;
+    auto iter1s = [&](const int t, auto S1c, const bool skip_t) -> bool {
+        constexpr int s1 = decltype(S1c)::value, s2 = (s1 + 1) % NS;
+        f16v c;
 #pragma unroll
-    for (int st = 0; st < NRW; ++st) row_sum(z, st);
-    tile_close();
+        for (int r = 0; r < 16; ++r) c[r] = 0.f;
+        // the largest aK of tile t+1 (read before this iteration's DMA)
+        const float akmax = ldKmax(s1);
+        h8 ya0[KS], ya1[KS];
+#pragma unroll
+        for (int ks = 0; ks < 2 && ks < KS; ++ks) ya0[ks] = ldY(s1, 0, ks);
+        constexpr int WA = NIW + RA;               // phase A steps: DMA loads, then row sums
+        // one code path for all four cases: the row sums and the lo-term
+        // MFMAs sit under wave-uniform branches of their own (two whole-phase
+        // copies per case would keep both cases' values live across the
+        // screen and cost a third of the registers)
+        dma_offsets_opaque();
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks + 2 < KS) ya0[ks + 2] = ldY(s1, 0, ks + 2);
+            // the first lo-term operands, read in any case (two LDS reads) so
+            // that phase B does not start on a read
+            if (ks >= KS - 2) ya1[ks - (KS - 2)] = ldY(s1, 1, ks - (KS - 2));
+            c = mfma(ya0[ks], tb[0][ks], c);
+#pragma unroll
+            for (int st = ks * WA / KS; st < (ks + 1) * WA / KS; ++st) {
+                if (st < NIW) dma(t + 2, s2, st);
+                else row_sum(z, st - NIW);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the screen: an upper bound of every exponent of tile t+1 per point
+        // (c1 > 0: the largest product, the tile's largest aK, the lo-term bound)
+        float cm = c[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) cm = fmaxf(cm, c[r]);
+        const float ub = fmaf(c1, xhalf_max(cm), akmax) + errz;
+        const bool out = __all(ub < m - smargin);
+        constexpr int WB = NRW - RA;               // phase B steps: the other row sums
+        if (!out) {
+            f4v akr[4];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                if (ks + 2 < KS) ya1[ks + 2] = ldY(s1, 1, ks + 2);
+                if (ks == KS - 1) {                // tile t+1's aK, for its exponents
+#pragma unroll
+                    for (int m4 = 0; m4 < 4; ++m4) akr[m4] = ldK(s1, m4);
+                }
+                c = mfma(ya1[ks], tb[0][ks], c);
+#pragma unroll
+                for (int st = ks * WB / KS; st < (ks + 1) * WB / KS; ++st) row_sum(z, RA + st);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (!skip_t) tile_close();
+            tile_exp(akr, c, z);
+        } else {
+#pragma unroll
+            for (int st = RA; st < NRW; ++st) row_sum(z, st);
+            if (!skip_t) tile_close();
+            // the next iteration's row sums of this tile add zeros
+#pragma unroll
+            for (int r = 0; r < 16; ++r) z[r] = -INFINITY;
+            if (GMX) gz = fmaxf(gz, ub);
+        }
+        group_end(t + 1);
+        __syncthreads();                           // also waits for this iteration's DMA
+        return out;
+    };
+    if (SCR) {
+        bool skip = false;
+        for (int t = 0; t + 1 < ntiles; t += 2) {
+            skip = iter1s(t, std::integral_constant<int, 1>{}, skip);
+            if (t + 2 >= ntiles) break;
+            skip = iter1s(t + 1, std::integral_constant<int, 0>{}, skip);
+        }
+        if (!skip) {
+#pragma unroll
+            for (int st = 0; st < NRW; ++st) row_sum(z, st);
+            tile_close();
+        }
+    } else {
+        for (int t = 0; t + 1 < ntiles; t += 2) {
+            iter1(t, std::integral_constant<int, 1>{});
+            if (t + 2 >= ntiles) break;
+            iter1(t + 1, std::integral_constant<int, 0>{});
+        }
+#pragma unroll
+        for (int st = 0; st < NRW; ++st) row_sum(z, st);
+        tile_close();
+    }
 
     // lanes j and j+32 hold the same point (different centre rows) and share m
     const float S1o = __shfl_xor(S1, 32), C1o = __shfl_xor(C1, 32);
@@ -447,15 +566,25 @@ __device__ __forceinline__ void e1_body(
 }
 
 template <int KS, int TT, bool TWO>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k_e1_mfma(
+// (three waves per SIMD where the registers allow: at most 168 each; with the
+// screen's branches the D <= 192 instantiation needs 174 and runs two)
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(KS <= 11 ? 3 : 2))) void k_e1_mfma(
     const _Float16 *__restrict__ T16, const float *__restrict__ tt32, long N, long Np,
     const _Float16 *__restrict__ Y16, const float *__restrict__ aK32, int Kp,
     const double *__restrict__ mscal, float beta, float it, int isDA, double logK,
     float *__restrict__ pP, float *__restrict__ blkThr, double *__restrict__ pm,
     double *__restrict__ logg, double *__restrict__ logh, double *__restrict__ partLog,
-    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd) {
-    e1_body<KS, TT, TWO>(blockIdx.x, threadIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal, beta, it, isDA, logK,
-                         pP, blkThr, pm, logg, logh, partLog, perm, gmx, bnd);
+    const int *__restrict__ perm, float *__restrict__ gmx, float *__restrict__ bnd,
+    const int *__restrict__ p1start, float smargin) {
+    // the sweep starts at this workgroup's home group (see dg_mixed_order)
+    // (kept a scalar: an integer remainder would go through the VALU and turn
+    // every DMA address of the sweep into a vector one)
+    const int s0 = p1start ? __builtin_amdgcn_readfirstlane(p1start[blockIdx.x]) & ~3 : 0;
+    const int t0 = (s0 >= 0 && s0 < Kp / KT) ? s0 : 0;
+    // (the screen pays where the cross term takes several MFMAs: D > 48)
+    e1_body<KS, TT, TWO, true, TT == 1 && KS >= 4>(blockIdx.x, threadIdx.x, T16, tt32, N, Np, Y16, aK32, Kp, mscal,
+                                        beta, it, isDA, logK, pP, blkThr, pm, logg, logh, partLog,
+                                        perm, gmx, bnd, t0, smargin);
 }
 
 // scal[0..2] = sums over the pass-1 workgroup partials (fixed tree order).
@@ -489,14 +618,24 @@ __global__ __launch_bounds__(1024) void k_sum_partlog(const double *__restrict__
     k_sum_partlog_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, part, nb, scal);
 }
 
-// Per-centre exponent term of pass 1: aK32[k] = -beta log2(e) |y_k|^2 / 2
-// (the same FP32 expression pass 2 uses), -inf for padding centres.
+// Per-centre exponent term of pass 1, -beta log2(e) |y_k|^2 / 2 (the same
+// FP32 expression pass 2 uses), -inf for padding centres, stored per 32-centre
+// tile in AKS floats: the tile's 32 values, their largest (pass 1's screen),
+// three -inf pads.
 __device__ __forceinline__ void k_make_ak_body(
     const int bx, const int by, const int bz, const int gx, const float *__restrict__ yy32,
     int K, int Kp, float beta, float *__restrict__ aK32) {
     const int k = bx * blockDim.x + threadIdx.x;
     const float cb = -0.5f * beta * kLog2e;
-    if (k < Kp) aK32[k] = (k < K) ? cb * yy32[k] : -INFINITY;
+    const float v = (k < K) ? cb * yy32[k] : -INFINITY;
+    const int tile = k / KT, q = k & (KT - 1);
+    float mx = v;                                  // every lane takes part in the exchange
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (k < Kp) {
+        aK32[(long)tile * AKS + q] = v;
+        if (q < 4) aK32[(long)tile * AKS + KT + q] = (q == 0) ? mx : -INFINITY;
+    }
 }
 
 __global__ void k_make_ak(const float *__restrict__ yy32, int K, int Kp, float beta,
@@ -657,8 +796,14 @@ __device__ __forceinline__ void e2_body(
         return *(const f4v *)&smem[s][OFF_P + 2 * (which * BP + 8 * m4 + 4 * h)];
     };
     // cross-term MFMA number m (m = (TT + 1) ks + which)
+    // HL: every hi-term MFMA first, then the lo terms -- the order of pass 1's
+    // screened sweep (exact data, D > 48), so that both passes form
+    // bit-identical exponents; otherwise hi and lo alternate per k step, as in
+    // pass 1 without the screen
+    constexpr bool HL = TT == 1 && KS >= 4;
+    auto xks = [&](int m) { return HL ? (m < KS ? m : m - KS) : m / (TT + 1); };
     auto xmfma = [&](int m, const h8 &a0, const h8 &a1, f16v &c) {
-        const int ks = m / (TT + 1), wq = m % (TT + 1);
+        const int ks = xks(m), wq = HL ? (m < KS ? 0 : 1) : m % (TT + 1);
         if (wq == 0) c = mfma(a0, yb[0][ks], c);
         else if (wq == 1) c = mfma(a0, yb[1][ks], c);
         else c = mfma(a1, yb[0][ks], c);
@@ -757,7 +902,7 @@ __device__ __forceinline__ void e2_body(
         for (int r = 0; r < 16; ++r) c[r] = 0.f;
 #pragma unroll
         for (int m = 0; m < NX; ++m) {
-            const int ks = m / (TT + 1);
+            const int ks = xks(m);
             xmfma(m, ldT(s, 0, ks), ldT(s, TT - 1, ks), c);
         }
         f4v pb[4];
@@ -850,8 +995,10 @@ __device__ __forceinline__ void e2_body(
             };
 #pragma unroll
             for (int m = 0; m < NX; ++m) {
-                const int ks = m / (TT + 1);
-                if (m % (TT + 1) == 0 && ks + 2 < KS) {
+                const int ks = xks(m);
+                if (HL) {                          // each half reads the data rows again
+                    if (m + 2 < NX) xa0[xks(m + 2)] = ldT(s1, 0, xks(m + 2));
+                } else if (m % (TT + 1) == 0 && ks + 2 < KS) {
                     xa0[ks + 2] = ldT(s1, 0, ks + 2);
                     if (TT == 2) xa1[ks + 2] = ldT(s1, 1, ks + 2);
                 }
@@ -1355,11 +1502,12 @@ struct MixedBufs {
     double *mscal = nullptr;
     double *partLog = nullptr;    // 3 per pass-1 workgroup
     float *blkThr = nullptr;      // per 32-point block: largest mask threshold
-    float *aK32 = nullptr;        // Kp: pass-1 per-centre exponent term
+    float *aK32 = nullptr;        // Kp/KT x AKS: pass-1 per-centre exponent terms per tile
     double *amaxPart = nullptr;   // 2 x kAmaxWG
     float *gmx = nullptr;         // Kp/CT x Np: pass-1 maximum exponent per centre group and point
     float *bnd = nullptr;         // Kp/CT x Np/BP: pass-2 exponent bound per group and block
     int2 *brange = nullptr;       // NC x Kp/CT x RS: pass-2 block runs per (chunk, group)
+    bool screen = true;           // pass 1 screens tiles (data exact in FP16)
     int *itab = nullptr;          // 8 x Kp/CT x ceil(NC/8): pass-2 work list per XCD
     int *icnt = nullptr;          // 8: its lengths
     long Np = 0;
@@ -1400,18 +1548,23 @@ void launch_e2(dg_ctx *c, MixedBufs *b, float beta, float it, long chunk, int KG
 template <int KS, int TT>
 void launch_mixed(dg_ctx *c, MixedBufs *b, float beta, float it, int two, int isDA,
                   double logK) {
+    // pass 1 leaves out tiles whose every element is below 2^-48 of its
+    // point's sums (exponents 48 / it below the running maximum)
+    const float smargin = b->screen ? 48.0f / it : INFINITY;
     hipLaunchKernelGGL(k_make_ak, dim3((unsigned)((b->Kp + 255) / 256)), dim3(256), 0, c->stream,
                        c->yy32, c->K, b->Kp, beta, b->aK32);
     if (two)
         hipLaunchKernelGGL((k_e1_mfma<KS, TT, true>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
                            b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
-                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd);
+                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd,
+                           c->p1start, smargin);
     else
         hipLaunchKernelGGL((k_e1_mfma<KS, TT, false>), dim3((unsigned)(b->Np / PT)), dim3(NTH), 0,
                            c->stream, (const _Float16 *)c->T16, c->tt32, c->N, b->Np, b->Y16,
                            b->aK32, b->Kp, b->mscal, beta, it, isDA, logK, c->pP32, b->blkThr,
-                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd);
+                           c->pm, c->logg, c->logh, b->partLog, c->perm, b->gmx, b->bnd,
+                           c->p1start, smargin);
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
     const long chunk = b->CL;
     const int KG = b->Kp / CT;
@@ -1518,10 +1671,17 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) void k
         const BInst &I = find_inst<false>(tab, list, nlist, item);
         if (!I.do_e) continue;
         // no register room for the group maxima in the batched KS = 8, split-data
-        // pass 1 (those instances get no bounds, and pass 2 visits every block)
-        e1_body<KS, TT, TWO, !(KS == 8 && TT == 2)>(item - I.item_e1, opaque_tid(), I.T16, I.tt32, I.N, I.Np, I.Y16, I.aK32, I.Kp,
-                             I.mscal, I.beta_f, I.it, I.isDA, I.logK, I.pP32, I.blkThr, I.pm,
-                             I.logg, I.logh, I.partLog, I.perm, I.gmx, I.bnd);
+        // pass 1 (those instances get no bounds, and pass 2 visits every block);
+        // the screen and the start tile as in k_e1_mfma, so that each instance
+        // takes exactly its single-instance steps
+        const long bx = item - I.item_e1;
+        const int s0 = I.p1start ? __builtin_amdgcn_readfirstlane(I.p1start[bx]) & ~3 : 0;
+        const int t0 = (s0 >= 0 && s0 < I.Kp / KT) ? s0 : 0;
+        const float smargin = I.screen ? 48.0f / I.it : INFINITY;
+        e1_body<KS, TT, TWO, !(KS == 8 && TT == 2), TT == 1 && KS >= 4>(
+            bx, opaque_tid(), I.T16, I.tt32, I.N, I.Np, I.Y16, I.aK32, I.Kp, I.mscal, I.beta_f, I.it,
+            I.isDA, I.logK, I.pP32, I.blkThr, I.pm, I.logg, I.logh, I.partLog, I.perm, I.gmx, I.bnd,
+            t0, smargin);
     }
 }
 
@@ -1853,7 +2013,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
         (rc = dalloc(c, &c->yy32, (size_t)b->Kp)) || (rc = dalloc(c, &b->mscal, 4)) ||
         (rc = dalloc(c, &b->partLog, (size_t)3 * (b->Np / PT))) ||
         (rc = dalloc(c, &b->blkThr, (size_t)(b->Np / BP))) ||
-        (rc = dalloc(c, &b->aK32, (size_t)b->Kp)) ||
+        (rc = dalloc(c, &b->aK32, (size_t)(b->Kp / KT) * AKS)) ||
         (rc = dalloc(c, &c->pP32, (size_t)2 * b->Np)) ||
         (rc = dalloc(c, &c->partRT32, (size_t)b->NC * c->K * c->D)) ||
         (rc = dalloc(c, &c->partG32, (size_t)b->NC * c->K)))
@@ -1863,6 +2023,7 @@ int dg_mixed_prepare_t(dg_ctx *c) {
     if ((rc = dalloc(c, &b->brange, (size_t)KG * b->NC * RS)) ||
         (rc = dalloc(c, &b->itab, 8 * KG * ((b->NC + 7) / 8))) || (rc = dalloc(c, &b->icnt, 8)))
         return rc;
+    b->screen = !env_off("DG_MIXED_SCREEN");
     if (!env_off("DG_MIXED_SKIP") &&
         ((rc = dalloc(c, &b->gmx, KG * b->Np)) || (rc = dalloc(c, &b->bnd, KG * (b->Np / BP)))))
         return rc;
@@ -1924,6 +2085,19 @@ int dg_mixed_order(dg_ctx *c) {
     if ((rc = dalloc(c, &c->perm, (size_t)Np))) return rc;
     DG_HIP(c, hipMemcpyAsync(c->perm, perm.data(), sizeof(int) * Np, hipMemcpyHostToDevice,
                              c->stream));
+    // each pass-1 workgroup starts its sweep at the centre group of its middle
+    // point's nearest centre, so that its running maxima are near their final
+    // values early and the screen leaves out most of the other tiles
+    const long nwg = Np / PT;
+    std::vector<int> st((size_t)nwg);
+    for (long w = 0; w < nwg; ++w) {
+        const long n = std::min(N - 1, w * PT + PT / 2);
+        st[(size_t)w] = key[(size_t)perm[(size_t)n]] / CT * (CT / KT);
+    }
+    if ((rc = dalloc(c, &c->p1start, (size_t)nwg))) return rc;
+    DG_HIP(c, hipMemcpyAsync(c->p1start, st.data(), sizeof(int) * nwg, hipMemcpyHostToDevice,
+                             c->stream));
+    DG_HIP(c, hipStreamSynchronize(c->stream));    // the host vectors go out of scope
     return build_t16(c, b);
 }
 
@@ -1984,6 +2158,8 @@ int dg_batch_fill_mixed(const dg_ctx *c, BInst *I) {
     I->gmx = gmx_ok ? b->gmx : nullptr;
     I->bnd = gmx_ok ? b->bnd : nullptr;
     I->perm = c->perm;
+    I->p1start = c->p1start;
+    I->screen = b->screen ? 1 : 0;
     I->brange = b->brange;
     I->skipBelow = skip_below(32 * ((16 * b->KS + 31) / 32) > c->D);
     I->mscal = b->mscal; I->partLog = b->partLog; I->amaxPart = b->amaxPart;

@@ -151,10 +151,17 @@ class _ShadowSparse(_Shadow):
 def test_mixed_shadowing_fp64_fingerprints():
     """The same on the K = 8000, D = 166 fingerprint configuration at
     N = 5000 (about 5300 iterations, 189 temperatures).  Across its phase
-    transitions the centres of the two runs move apart transiently (W, Y up
-    to ~1e-2 relative) and come back together: stated tolerances llh 2e-4,
-    -F and beta 2e-4 over the run, W 5e-2 and Y 3e-2 (compared every 25
-    M-steps); at the end llh and beta 5e-6, W and Y 5e-5."""
+    transitions the centres of the two runs move apart for a while and come
+    back together.  How far and for how long depends on where the last bits
+    of the sums fall, not on the size of the per-step error.  Two
+    bit-different mixed builds with the same per-step errors measured
+    (median llh 2.3e-6 and 2.4e-6, beta 1.7e-5 and 1.8e-5, W 1.6e-5 and
+    1.7e-5) reached W 8e-3 and 0.25 at worst.  So the stated tolerances are
+    on the typical step, median and 90th percentile: llh 1e-5 and 5e-5, -F
+    2e-6 and 1e-5, beta 5e-5 and 2e-4, W and Y (every 25 M-steps) 1e-4 and
+    2e-3.  The worst step is only bounded against a runaway: llh and -F
+    5e-3, beta 1e-3, W and Y 0.5.  At the end of the run: llh and beta 5e-6,
+    W and Y 5e-5."""
     T = datagen.fingerprints(100_000, 166, 7)[:5000].astype(np.float64)
     X, FI = datagen.latent_basis(3, 8000, 125, 1.0)
     W0, Y0, b0 = datagen.pca_init(T, FI, X)
@@ -176,7 +183,17 @@ def test_mixed_shadowing_fp64_fingerprints():
     _say(f"  {r.nloop} iterations; max rel err llh {llh.max():.2e} -F {nfree.max():.2e} "
          f"beta {beta.max():.2e} W {np.nanmax(R[:, 7]):.2e} Y {np.nanmax(R[:, 8]):.2e}; "
          f"end llh {llh[-1]:.2e} beta {beta[-1]:.2e} W {eW:.2e} Y {eY:.2e}")
+    for q in (50, 90, 95, 99):
+        _say(f"  p{q}: llh {np.percentile(llh, q):.2e} -F {np.percentile(nfree, q):.2e} "
+             f"beta {np.percentile(beta, q):.2e} W {np.nanpercentile(R[:, 7], q):.2e} "
+             f"Y {np.nanpercentile(R[:, 8], q):.2e}")
     assert r.nloop > 1000
-    assert llh.max() < 2e-4 and nfree.max() < 2e-4 and beta.max() < 2e-4
-    assert np.nanmax(R[:, 7]) < 5e-2 and np.nanmax(R[:, 8]) < 3e-2
+    med = lambda x: np.nanpercentile(x, 50)
+    p90 = lambda x: np.nanpercentile(x, 90)
+    assert med(llh) < 1e-5 and med(nfree) < 2e-6 and med(beta) < 5e-5
+    assert med(R[:, 7]) < 1e-4 and med(R[:, 8]) < 1e-4
+    assert p90(llh) < 5e-5 and p90(nfree) < 1e-5 and p90(beta) < 2e-4
+    assert p90(R[:, 7]) < 2e-3 and p90(R[:, 8]) < 2e-3
+    assert llh.max() < 5e-3 and nfree.max() < 5e-3 and beta.max() < 1e-3
+    assert np.nanmax(R[:, 7]) < 0.5 and np.nanmax(R[:, 8]) < 0.5
     assert llh[-1] < 5e-6 and beta[-1] < 5e-6 and eW < 5e-5 and eY < 5e-5

@@ -9,11 +9,16 @@ are common.  Both are exact in the following sense, checked here:
     zeros -- while the skipping run visits fewer pairs once the
     responsibilities concentrate;
   - ordering: with the points in input order (DG_MIXED_ORDER=0) the results
-    from the same state differ only by the order of the FP32/FP64 sums (llh
-    1e-12; G, R T, W, Y and beta 1e-6), and per-point outputs, which use the
-    input order, agree;
+    from the same state differ only by the order of the FP32/FP64 sums (pass
+    1 also starts each point tile's sweep at its home centre group in the
+    ordered layout, and its screen leaves out other terms below 2^-48 of a
+    point's sums: llh 1e-8; G, R T, W, Y and beta 1e-6), and per-point
+    outputs, which use the input order, agree;
   - several annealing steps along a trajectory where the skipped fraction
-    grows stay bit-identical to the no-skip run.
+    grows stay bit-identical to the no-skip run;
+  - pass 1's screen of tiles far below each point's running maximum
+    (DG_MIXED_SCREEN=0 turns it off) changes the results at most in their
+    last bits.
 The workload: binary fingerprints (50 prototypes, flip 0.05) at D = 166 and
 K = 1728 (L = 3), the same with empty and two-bit points mixed in (points
 whose pass-2 offset is positive), and Gaussian-mixture data at D = 32 (no
@@ -99,17 +104,40 @@ def test_skipping_is_bit_identical(kind):
         assert fr[-1] < 0.8 and fr[-1] < fr[0], fr
 
 
+@pytest.mark.parametrize("kind", ["fp", "fp_sparse"])
+def test_pass1_screen_leaves_out_only_negligible_terms(kind):
+    """Pass 1 leaves out tiles whose every exponent is more than 48 / it below
+    its point's running maximum (a bound from the hi-term product): each
+    omitted term weighs less than 2^-48 of its point's sums, so E- and
+    M-steps with the screen (default) and without it (DG_MIXED_SCREEN=0)
+    agree to the last bits -- mostly bit for bit; where a compensated sum
+    rounds the other way, llh within 1e-12 and G, R T, W, Y and beta within
+    1e-9 of their maxima over five steps."""
+    _say(f"case: pass-1 screen, {kind}")
+    prob = _problem(kind)
+    a = _run(prob, {}, 5, 2.0)
+    b = _run(prob, {"DG_MIXED_SCREEN": "0"}, 5, 2.0)
+    for i, (x, y) in enumerate(zip(a, b)):
+        rel = lambda u, v: float(np.abs(u - v).max() / max(np.abs(v).max(), 1e-300))
+        d = [rel(x[k], y[k]) for k in (2, 3, 5, 6)]
+        dl, db = abs(x[0] - y[0]) / abs(y[0]), abs(x[4] - y[4]) / y[4]
+        _say(f"  step {i}: llh {dl:.1e} G {d[0]:.1e} RT {d[1]:.1e} W {d[2]:.1e} Y {d[3]:.1e} "
+             f"beta {db:.1e}")
+        assert x[1] == y[1] and dl <= 1e-12 and db <= 1e-9, (i, dl, db)
+        assert max(d) <= 1e-9, (i, d)
+
+
 def test_point_order_changes_only_the_sum_order():
-    """From the same state: llh to 1e-12 (per-point values are identical,
-    only the FP64 sum over points is grouped differently), G and R T to 1e-6
-    of their maxima (FP32 chunk sums over other points), and the M-step's W,
-    Y and beta to 1e-6."""
+    """From the same state: llh to 1e-8 (each point's FP32 sums over centres
+    run in another order, and the FP64 sum over points is grouped
+    differently), G and R T to 1e-6 of their maxima (FP32 chunk sums over
+    other points), and the M-step's W, Y and beta to 1e-6."""
     _say("case: point ordering vs input order")
     prob = _problem("fp")
     x = _run(prob, {}, 1, 2.0)[0]
     y = _run(prob, {"DG_MIXED_ORDER": "0"}, 1, 2.0)[0]
     _say(f"  llh {x[0]:.15g} vs {y[0]:.15g}; beta {x[4]:.15g} vs {y[4]:.15g}")
-    assert x[0] == pytest.approx(y[0], rel=1e-12)
+    assert x[0] == pytest.approx(y[0], rel=1e-8)
     for k in (2, 3, 5, 6):
         assert np.abs(x[k] - y[k]).max() <= 1e-6 * np.abs(y[k]).max(), k
     assert x[4] == pytest.approx(y[4], rel=1e-6)
@@ -138,4 +166,6 @@ def test_per_point_outputs_keep_input_order():
                     os.environ.pop(k, None)
                 else:
                     os.environ[k] = v
-    assert np.abs(res[0] - res[1]).max() <= 1e-6, np.abs(res[0] - res[1]).max()
+    # one M-step from states that differ by the order of the sums (W, Y about
+    # 5e-7): the posterior means move by about 1e-6 (measured 1.1e-6)
+    assert np.abs(res[0] - res[1]).max() <= 1e-5, np.abs(res[0] - res[1]).max()

@@ -101,7 +101,9 @@ def test_checker_sees_an_unfenced_read():
 def test_pass1_accumulator_in_vgprs(asm):
     """Pass 1 reads its accumulator with VALU FMAs: it must live in ordinary
     VGPRs (no AGPRs, no v_accvgpr_read), and every instantiation must still fit
-    three waves per SIMD (at most 168 registers)."""
+    three waves per SIMD (at most 168 registers) -- except the screened
+    single-instance kernel at D <= 192 (KS = 12), which is allowed two (at
+    most 256)."""
     text = "\n".join(asm)
     names = sorted(set(re.findall(r"^(_Z\w*kb?_e1_mfma\w*):", text, re.M)))
     assert len(names) >= 30, "pass-1 instantiations not found"
@@ -110,7 +112,8 @@ def test_pass1_accumulator_in_vgprs(asm):
         vgpr = re.search(r"\.set " + name + r"\.num_vgpr, (\d+)", text)
         assert agpr and vgpr, name
         assert int(agpr.group(1)) == 0, (name, agpr.group(1))
-        assert int(vgpr.group(1)) <= 168, (name, vgpr.group(1))
+        cap = 256 if re.search(r"\dk_e1_mfmaILi12ELi1E", name) else 168
+        assert int(vgpr.group(1)) <= cap, (name, vgpr.group(1))
     for stem in ("k_e1_mfma", "kb_e1_mfma"):
         for name, body in _kernels(asm, stem).items():
             assert not any(x.startswith("v_accvgpr_read") for x in body), name
@@ -174,9 +177,12 @@ def _e2_lds_reads_expected(KS, TT):
     """16-byte LDS reads of one pass-2 instantiation: the prologue block
     (KS TT data-row operands, 4 pB and 4 mask-threshold vectors), three
     unrolled loop iterations (the same plus the 2 DB TT transposed-row
-    operands of the R T product) and the epilogue block's R T operands."""
+    operands of the R T product) and the epilogue block's R T operands.
+    With exact data and KS >= 4 the loop takes every hi-term MFMA before the
+    lo-term ones and reads each data-row operand once per half (2 KS)."""
     DB = (16 * KS + 31) // 32
-    return (KS * TT + 8) + 3 * (KS * TT + 2 * DB * TT + 8) + 2 * DB * TT
+    loop_rows = 2 * KS if (TT == 1 and KS >= 4) else KS * TT
+    return (KS * TT + 8) + 3 * (loop_rows + 2 * DB * TT + 8) + 2 * DB * TT
 
 
 def test_pass2_reads_every_operand(asm):
