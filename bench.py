@@ -103,18 +103,23 @@ def run_steps(eng, steps, visited=None):
     (centre group, block) pairs that E-pass 2 computed."""
     llh = []
     phase = np.zeros(4)
+    probe = 0.0                    # host time of the visited-pairs read-back, not part of a step
     t0 = time.perf_counter()
     for _ in range(steps):
         l, _, _ = eng.estep(True, TEMP)
         ph = eng.timing()
         if visited is not None:
+            # the E-step has returned (the stream is idle): this read-back only
+            # adds host time, which is taken out of the step time
+            tp = time.perf_counter()
             v, t = eng.pass2_blocks()
             visited.append(v / t)
+            probe += time.perf_counter() - tp
         eng.mstep()
         ph[3] = eng.timing()[3]
         phase += ph
         llh.append(l)
-    return time.perf_counter() - t0, llh, phase
+    return time.perf_counter() - t0 - probe, llh, phase
 
 
 def _oracle_time(O, T, FI, Y0, b0, n, threads, solver, iters=2):
