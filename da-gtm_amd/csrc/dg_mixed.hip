@@ -47,12 +47,13 @@ constexpr int BP = 32;            // pass-2 points per block
 constexpr int AKS = KT + 4;       // pass-1 aK per tile: 32 values, the largest, 3 pads
 constexpr int PAD = 8;            // LDS row padding (halves)
 // Pass 2 visits, per (point chunk, centre group) pair, at most MAXR runs of
-// adjacent blocks; dead gaps of at most RGAP blocks stay inside a run.  The
-// pair's entry in the run table is RS int2: {runs, blocks}, then the runs
-// {first block within the chunk, count}.
-constexpr int MAXR = 15;
+// adjacent blocks; dead gaps of at most RGAP blocks stay inside a run (none:
+// a run's restart costs less than a dead block).  The pair's entry in the run
+// table is RS int2: {runs, blocks}, then the runs {first block within the
+// chunk, count}.
+constexpr int MAXR = 31;
 constexpr int RS = MAXR + 1;
-constexpr int RGAP = 3;
+constexpr int RGAP = 0;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kZmin = -1075.0f;  // log2 of the FP64 exp underflow point
 constexpr float kRLog2 = 14.0f;    // R is carried as R * 2^14 in FP16
