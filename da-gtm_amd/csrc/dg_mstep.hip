@@ -638,86 +638,142 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
     k_chol_lds_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, A0, L, n, ridge, scal);
 }
 
-// W[:, c0:c0+32] from L y = b, L^T x = y (n <= 126): L in LDS, thread
-// (c, pr) of 32 x 32 keeps rows p = pr + 32 a of right-hand side c in
-// registers.  Step i: the owner of row i finishes x_i and publishes it in an
-// LDS ping-pong buffer; after one barrier every thread updates its rows after
-// (forward) or before (backward) row i.
+// W[:, c0:c0+NT/32] from L y = b, L^T x = y (n <= 126), NT threads: L in LDS
+// (rows at an odd stride, so that a column read by a wave spreads over the
+// banks); wave w solves right-hand sides c0 + 2 w and c0 + 2 w + 1 on its own, lane
+// l keeping rows l and l + 64 of both, and their 1 / L_pp, in registers.
+// Step i: x_i and 1 / L_ii go from the lane that owns row i to the whole wave
+// by readlane, with no barrier and no LDS access on the way, and every lane
+// updates its rows after (forward) or before (backward) row i with the L
+// elements it read from LDS during the step before.  The arithmetic is that
+// of the plain substitution, x_i = x_i (1 / L_ii), then one FMA per later
+// (earlier) row, in the same order for every row.
+template <int NT>
 __device__ __forceinline__ void k_trsm_lds_body(
     const int bx, const int by, const int bz, const int gx, const double *__restrict__ L, int n,
     const double *__restrict__ B, int nrhs, double *__restrict__ W,
     const double *__restrict__ scal) {
     if (scal[kScalSolveOK] != 1.0) return;
-    __shared__ double Ls[kLdsN * kLdsN];
-    __shared__ double rd[kLdsN];
-    __shared__ double xb[2][32];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int c = tid & 31, pr = tid >> 5;
-    const int c0 = bx * 32;
-    const bool live = c0 + c < nrhs;
-    for (int e = tid; e < n * n; e += nt) {
-        const int i = e / n, k = e % n;
-        if (k <= i) Ls[e] = L[e];                    // lower triangle only
-    }
-    for (int i = tid; i < n; i += nt) rd[i] = 1.0 / L[(long)i * n + i];
-    double x[4];
+    constexpr int LDL = kLdsN | 1;
+    __shared__ double Ls[kLdsN * LDL];
+    const int tid = threadIdx.x, nt = NT;
+    const int l = tid & 63, w = tid >> 6;
+    const int ld = n | 1;                          // row stride in LDS, at most LDL
+    // the lower triangle: all of a thread's loads are in flight before its
+    // first LDS store
+    constexpr int FILL = (kLdsN * kLdsN + NT - 1) / NT;
+    {
+        double v[FILL];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int p = pr + 32 * a;
-        x[a] = (live && p < n) ? B[(long)p * nrhs + c0 + c] : 0.0;
+        for (int f = 0; f < FILL; ++f) {
+            const int e = tid + f * nt, i = e / n, k = e - i * n;
+            v[f] = (e < n * n && k <= i) ? L[e] : 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < FILL; ++f) {
+            const int e = tid + f * nt, i = e / n, k = e - i * n;
+            if (e < n * n && k <= i) Ls[i * ld + k] = v[f];
+        }
+    }
+    const int c0 = bx * (NT / 32);
+    double x[2][2], rdv[2];
+    bool live[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int p = l + 64 * a;
+        rdv[a] = (p < n) ? 1.0 / L[(long)p * n + p] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int col = c0 + 2 * w + q;
+        live[q] = col < nrhs;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int p = l + 64 * a;
+            x[q][a] = (live[q] && p < n) ? B[(long)p * nrhs + col] : 0.0;
+        }
     }
     __syncthreads();
+    // lane `src`'s v on every lane (src is the same for the whole wave)
+    auto bcast = [](double v, int src) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+        return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
+    };
+    // x_i from its owner, finished
+    auto pivot = [&](int i, double (&xi)[2]) {
+        const int ai = i >> 6, li = i & 63;
+        const double r = bcast(ai ? rdv[1] : rdv[0], li);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) xi[q] = bcast(ai ? x[q][1] : x[q][0], li) * r;
+    };
+    // the L elements of the next step (rows p < n only: the others lie
+    // outside the LDS rows written)
+    double lpn[2] = {0.0, 0.0};
+    auto next_l = [&](int off, int pstride) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int p = l + 64 * a;
+            if (p < n) lpn[a] = Ls[p * pstride + off];
+        }
+    };
+    next_l(0, ld);                                 // column 0
     for (int i = 0; i < n; ++i) {                  // forward
-        if (pr == (i & 31)) {
-            const int ai = i >> 5;
-            double xi = x[0];
+        const double lp[2] = {lpn[0], lpn[1]};
+        if (i + 1 < n) next_l(i + 1, ld);          // column i + 1
+        double xi[2];
+        pivot(i, xi);
 #pragma unroll
-            for (int a = 1; a < 4; ++a) xi = (ai == a) ? x[a] : xi;
-            xi *= rd[i];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) x[a] = (ai == a) ? xi : x[a];
-            xb[i & 1][c] = xi;
-        }
-        __syncthreads();
-        const double xi = xb[i & 1][c];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int p = pr + 32 * a;
-            if (p > i && p < n) x[a] = fma(-Ls[p * n + i], xi, x[a]);
+        for (int a = 0; a < 2; ++a) {
+            const int p = l + 64 * a;
+            if (p == i) {
+                x[0][a] = xi[0];
+                x[1][a] = xi[1];
+            } else if (p > i && p < n) {
+                x[0][a] = fma(-lp[a], xi[0], x[0][a]);
+                x[1][a] = fma(-lp[a], xi[1], x[1][a]);
+            }
         }
     }
-    __syncthreads();                               // xb is reused from here on
+    next_l((n - 1) * ld, 1);                       // row n - 1
     for (int i = n - 1; i >= 0; --i) {             // backward with L^T
-        if (pr == (i & 31)) {
-            const int ai = i >> 5;
-            double xi = x[0];
+        const double lp[2] = {lpn[0], lpn[1]};
+        if (i > 0) next_l((i - 1) * ld, 1);        // row i - 1
+        double xi[2];
+        pivot(i, xi);
 #pragma unroll
-            for (int a = 1; a < 4; ++a) xi = (ai == a) ? x[a] : xi;
-            xi *= rd[i];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) x[a] = (ai == a) ? xi : x[a];
-            xb[i & 1][c] = xi;
-        }
-        __syncthreads();
-        const double xi = xb[i & 1][c];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int p = pr + 32 * a;
-            if (p < i) x[a] = fma(-Ls[i * n + p], xi, x[a]);
+        for (int a = 0; a < 2; ++a) {
+            const int p = l + 64 * a;
+            if (p == i) {
+                x[0][a] = xi[0];
+                x[1][a] = xi[1];
+            } else if (p < i) {
+                x[0][a] = fma(-lp[a], xi[0], x[0][a]);
+                x[1][a] = fma(-lp[a], xi[1], x[1][a]);
+            }
         }
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int p = pr + 32 * a;
-        if (live && p < n) W[(long)p * nrhs + c0 + c] = x[a];
+    for (int q = 0; q < 2; ++q) {
+        const int col = c0 + 2 * w + q;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int p = l + 64 * a;
+            if (live[q] && p < n) W[(long)p * nrhs + col] = x[q][a];
+        }
     }
 }
 
-__global__ __launch_bounds__(1024) void k_trsm_lds(const double *__restrict__ L, int n,
-                                                   const double *__restrict__ B, int nrhs,
-                                                   double *__restrict__ W,
-                                                   const double *__restrict__ scal) {
-    k_trsm_lds_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, L, n, B, nrhs, W, scal);
+// one wave per SIMD: the substitution is issue-bound, so the right-hand
+// sides go 8 to a workgroup over more CUs
+constexpr int kTrsmNT = 256;
+__global__ __launch_bounds__(kTrsmNT) void k_trsm_lds(const double *__restrict__ L, int n,
+                                                      const double *__restrict__ B, int nrhs,
+                                                      double *__restrict__ W,
+                                                      const double *__restrict__ scal) {
+    k_trsm_lds_body<kTrsmNT>(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, L, n, B, nrhs, W,
+                             scal);
 }
 
 // Y = FI W into the padded FP64 layout (K x ld, columns >= D untouched):
@@ -874,7 +930,7 @@ __global__ __launch_bounds__(1024) void kb_trsm_lds(const BInst *__restrict__ ta
     const int2 m = map[blockIdx.x];
     const BInst &I = tab[m.x];
     if (!I.do_m) return;
-    k_trsm_lds_body(m.y, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
+    k_trsm_lds_body<1024>(m.y, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
 }
 
 __global__ __launch_bounds__(1024) void kb_chol_solve(const BInst *__restrict__ tab,
@@ -1031,7 +1087,8 @@ int dg_mstep_run(dg_ctx *c, double jitter) {
         hipLaunchKernelGGL(k_chol_lds, dim3(1), dim3(1024), 0, c->stream, c->A0, c->A, M1, ridge,
                            c->scal);
         DG_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + 31) / 32)), dim3(1024), 0, c->stream,
+        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + kTrsmNT / 32 - 1) / (kTrsmNT / 32))),
+                           dim3(kTrsmNT), 0, c->stream,
                            c->A, M1, c->Bm, D, c->W, c->scal);
     } else {
         int rc = blocked_chol_solve(c, ridge, jitter);
