@@ -510,11 +510,11 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-constexpr int kPB = 16;       // panel width of the blocked Cholesky
+constexpr int kPB = 8;        // panel width of the blocked Cholesky
 
 // Cholesky of A0 + (ridge + jitter) I with the matrix resident in LDS (one
 // workgroup, n <= 126); same pivots and retries as k_chol_solve.  Blocked,
-// right-looking: wave 0 factors each 16-column panel in registers (lane r
+// right-looking: wave 0 factors each 8-column panel in registers (lane r
 // holds rows p0 + r and p0 + r + 64; the pivot row and the L entries of later
 // panel columns are broadcast with readlane, so the panel needs no barrier),
 // then all waves apply the panel to the trailing lower triangle with 4 x 4
