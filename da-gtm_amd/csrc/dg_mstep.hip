@@ -638,17 +638,17 @@ __global__ __launch_bounds__(1024) void k_chol_lds(const double *__restrict__ A0
     k_chol_lds_body(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, A0, L, n, ridge, scal);
 }
 
-// W[:, c0:c0+NT/32] from L y = b, L^T x = y (n <= 126), NT threads: L in LDS
-// (rows at an odd stride, so that a column read by a wave spreads over the
-// banks); wave w solves right-hand sides c0 + 2 w and c0 + 2 w + 1 on its own, lane
-// l keeping rows l and l + 64 of both, and their 1 / L_pp, in registers.
+// W[:, c0:c0+CPW NT/64] from L y = b, L^T x = y (n <= 126), NT threads: L in
+// LDS (rows at an odd stride, so that a column read by a wave spreads over the
+// banks); wave w solves the CPW right-hand sides c0 + CPW w + q on its own, lane
+// l keeping rows l and l + 64 of each, and their 1 / L_pp, in registers.
 // Step i: x_i and 1 / L_ii go from the lane that owns row i to the whole wave
 // by readlane, with no barrier and no LDS access on the way, and every lane
 // updates its rows after (forward) or before (backward) row i with the L
 // elements it read from LDS during the step before.  The arithmetic is that
 // of the plain substitution, x_i = x_i (1 / L_ii), then one FMA per later
 // (earlier) row, in the same order for every row.
-template <int NT>
+template <int NT, int CPW>
 __device__ __forceinline__ void k_trsm_lds_body(
     const int bx, const int by, const int bz, const int gx, const double *__restrict__ L, int n,
     const double *__restrict__ B, int nrhs, double *__restrict__ W,
@@ -675,17 +675,17 @@ __device__ __forceinline__ void k_trsm_lds_body(
             if (e < n * n && k <= i) Ls[i * ld + k] = v[f];
         }
     }
-    const int c0 = bx * (NT / 32);
-    double x[2][2], rdv[2];
-    bool live[2];
+    const int c0 = bx * (CPW * NT / 64);
+    double x[CPW][2], rdv[2];
+    bool live[CPW];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         const int p = l + 64 * a;
         rdv[a] = (p < n) ? 1.0 / L[(long)p * n + p] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int col = c0 + 2 * w + q;
+    for (int q = 0; q < CPW; ++q) {
+        const int col = c0 + CPW * w + q;
         live[q] = col < nrhs;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -702,11 +702,11 @@ __device__ __forceinline__ void k_trsm_lds_body(
         return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
     };
     // x_i from its owner, finished
-    auto pivot = [&](int i, double (&xi)[2]) {
+    auto pivot = [&](int i, double (&xi)[CPW]) {
         const int ai = i >> 6, li = i & 63;
         const double r = bcast(ai ? rdv[1] : rdv[0], li);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) xi[q] = bcast(ai ? x[q][1] : x[q][0], li) * r;
+        for (int q = 0; q < CPW; ++q) xi[q] = bcast(ai ? x[q][1] : x[q][0], li) * r;
     };
     // the L elements of the next step (rows p < n only: the others lie
     // outside the LDS rows written)
@@ -722,17 +722,17 @@ __device__ __forceinline__ void k_trsm_lds_body(
     for (int i = 0; i < n; ++i) {                  // forward
         const double lp[2] = {lpn[0], lpn[1]};
         if (i + 1 < n) next_l(i + 1, ld);          // column i + 1
-        double xi[2];
+        double xi[CPW];
         pivot(i, xi);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int p = l + 64 * a;
             if (p == i) {
-                x[0][a] = xi[0];
-                x[1][a] = xi[1];
+#pragma unroll
+                for (int q = 0; q < CPW; ++q) x[q][a] = xi[q];
             } else if (p > i && p < n) {
-                x[0][a] = fma(-lp[a], xi[0], x[0][a]);
-                x[1][a] = fma(-lp[a], xi[1], x[1][a]);
+#pragma unroll
+                for (int q = 0; q < CPW; ++q) x[q][a] = fma(-lp[a], xi[q], x[q][a]);
             }
         }
     }
@@ -740,23 +740,23 @@ __device__ __forceinline__ void k_trsm_lds_body(
     for (int i = n - 1; i >= 0; --i) {             // backward with L^T
         const double lp[2] = {lpn[0], lpn[1]};
         if (i > 0) next_l((i - 1) * ld, 1);        // row i - 1
-        double xi[2];
+        double xi[CPW];
         pivot(i, xi);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int p = l + 64 * a;
             if (p == i) {
-                x[0][a] = xi[0];
-                x[1][a] = xi[1];
+#pragma unroll
+                for (int q = 0; q < CPW; ++q) x[q][a] = xi[q];
             } else if (p < i) {
-                x[0][a] = fma(-lp[a], xi[0], x[0][a]);
-                x[1][a] = fma(-lp[a], xi[1], x[1][a]);
+#pragma unroll
+                for (int q = 0; q < CPW; ++q) x[q][a] = fma(-lp[a], xi[q], x[q][a]);
             }
         }
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int col = c0 + 2 * w + q;
+    for (int q = 0; q < CPW; ++q) {
+        const int col = c0 + CPW * w + q;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int p = l + 64 * a;
@@ -765,15 +765,15 @@ __device__ __forceinline__ void k_trsm_lds_body(
     }
 }
 
-// one wave per SIMD: the substitution is issue-bound, so the right-hand
-// sides go 8 to a workgroup over more CUs
-constexpr int kTrsmNT = 256;
+// one wave per SIMD and one right-hand side per wave: the substitution is
+// issue-bound, so the right-hand sides go 4 to a workgroup over more CUs
+constexpr int kTrsmNT = 256, kTrsmCPW = 1;
 __global__ __launch_bounds__(kTrsmNT) void k_trsm_lds(const double *__restrict__ L, int n,
                                                       const double *__restrict__ B, int nrhs,
                                                       double *__restrict__ W,
                                                       const double *__restrict__ scal) {
-    k_trsm_lds_body<kTrsmNT>(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, L, n, B, nrhs, W,
-                             scal);
+    k_trsm_lds_body<kTrsmNT, kTrsmCPW>(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, L, n, B,
+                                       nrhs, W, scal);
 }
 
 // Y = FI W into the padded FP64 layout (K x ld, columns >= D untouched):
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(1024) void kb_trsm_lds(const BInst *__restrict__ ta
     const int2 m = map[blockIdx.x];
     const BInst &I = tab[m.x];
     if (!I.do_m) return;
-    k_trsm_lds_body<1024>(m.y, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
+    k_trsm_lds_body<1024, 2>(m.y, 0, 0, 0, I.A, I.M1, I.Bm, I.D, I.W, I.scal);
 }
 
 __global__ __launch_bounds__(1024) void kb_chol_solve(const BInst *__restrict__ tab,
@@ -1087,7 +1087,8 @@ int dg_mstep_run(dg_ctx *c, double jitter) {
         hipLaunchKernelGGL(k_chol_lds, dim3(1), dim3(1024), 0, c->stream, c->A0, c->A, M1, ridge,
                            c->scal);
         DG_HIP(c, hipGetLastError());
-        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + kTrsmNT / 32 - 1) / (kTrsmNT / 32))),
+        constexpr int cols = kTrsmCPW * kTrsmNT / 64;     // right-hand sides per workgroup
+        hipLaunchKernelGGL(k_trsm_lds, dim3((unsigned)((D + cols - 1) / cols)),
                            dim3(kTrsmNT), 0, c->stream,
                            c->A, M1, c->Bm, D, c->W, c->scal);
     } else {
