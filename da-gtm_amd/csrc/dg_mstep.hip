@@ -434,8 +434,14 @@ __device__ __forceinline__ void k_partAB_body(
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int kb = k0; kb < k1; kb += GK) {
-        for (int e = tid; e < GK * GT; e += 256) {
+    // the next step's operands are loaded into registers while this step's
+    // are multiplied from LDS (four elements of each per thread)
+    static_assert(GK * GT == 4 * 256, "four staged elements per thread");
+    double ra[4], rb[4];
+    auto gload = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
             const int kk = e / GT, col = e % GT, k = kb + kk;
             double av = 0.0, bv = 0.0;
             if (k < k1) {
@@ -444,10 +450,20 @@ __device__ __forceinline__ void k_partAB_body(
                 if (jj < M1) bv = G[k] * FI[(long)k * M1 + jj];
                 else if (jj < NCOL) bv = RT[(long)k * D + (jj - M1)];
             }
-            As[kk][col] = av;
-            Bs[kk][col] = bv;
+            ra[u] = av;
+            rb[u] = bv;
+        }
+    };
+    if (k0 < k1) gload(k0);
+    for (int kb = k0; kb < k1; kb += GK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[e / GT][e % GT] = ra[u];
+            Bs[e / GT][e % GT] = rb[u];
         }
         __syncthreads();
+        if (kb + GK < k1) gload(kb + GK);
 #pragma unroll
         for (int kk = 0; kk < GK; ++kk) {
             double av[4], bv[4];
@@ -789,16 +805,32 @@ __device__ __forceinline__ void k_phiw_body(
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int ib = 0; ib < M1; ib += GK) {
-        for (int e = tid; e < GK * GT; e += 256) {
+    // the next step's operands are loaded into registers while this step's
+    // are multiplied from LDS (four elements of each per thread)
+    static_assert(GK * GT == 4 * 256, "four staged elements per thread");
+    double ra[4], rb[4];
+    auto gload = [&](int ib) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
             const int row = e / GK, ii = e % GK;          // FI rows read along i
             const int k = k0 + row, i = ib + ii;
-            As[ii][row] = (k < K && i < M1) ? FI[(long)k * M1 + i] : 0.0;
+            ra[u] = (k < K && i < M1) ? FI[(long)k * M1 + i] : 0.0;
             const int kk = e / GT, col = e % GT;          // W rows read along d
             const int i2 = ib + kk, d = d0 + col;
-            Bs[kk][col] = (i2 < M1 && d < D) ? W[(long)i2 * D + d] : 0.0;
+            rb[u] = (i2 < M1 && d < D) ? W[(long)i2 * D + d] : 0.0;
+        }
+    };
+    gload(0);
+    for (int ib = 0; ib < M1; ib += GK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[e % GK][e / GK] = ra[u];
+            Bs[e / GT][e % GT] = rb[u];
         }
         __syncthreads();
+        if (ib + GK < M1) gload(ib + GK);
 #pragma unroll
         for (int kk = 0; kk < GK; ++kk) {
             double av[4], bv[4];
