@@ -1112,11 +1112,16 @@ __device__ __forceinline__ void block_range_body(const int lid, const float *__r
         if ((threadIdx.x & 63) == 0) s_live[w] = bal;
         __syncthreads();
         if (threadIdx.x == 0) {
+            // one stretch of adjacent live blocks at a time: from its first
+            // block s, the stretch runs to the first dead block after it
             for (int x = 0; x < NTH / 64; ++x) {
                 unsigned long long m = s_live[x];
                 while (m) {
-                    const int b = q0 + 64 * x + __builtin_ctzll(m);
-                    m &= m - 1;
+                    const int s0 = __builtin_ctzll(m);
+                    const unsigned long long dead = ~m >> s0;
+                    const int len = dead ? __builtin_ctzll(dead) : 64 - s0;
+                    const int b = q0 + 64 * x + s0;
+                    m = (s0 + len < 64) ? m & (~0ull << (s0 + len)) : 0ull;
                     if (cs < 0) {
                         cs = b;
                     } else if (b - cl - 1 > RGAP && nr < MAXR - 1) {
@@ -1125,7 +1130,7 @@ __device__ __forceinline__ void block_range_body(const int lid, const float *__r
                         ++nr;
                         cs = b;
                     }
-                    cl = b;
+                    cl = b + len - 1;
                 }
             }
         }
