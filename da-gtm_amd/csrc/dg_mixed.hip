@@ -1226,8 +1226,17 @@ __device__ __forceinline__ void k_reduce32_body(
         if (k0 + r < K && d0 + e < D) RT[(long)(k0 + r) * D + d0 + e] = tile[r][e];
     }
     if (by == 0 && t < RK && k < K) {
+        // RU loads in flight, then their sums in chunk order (as for R T)
         double g = 0.0;
-        for (int c = 0; c < NC; ++c) g += live(c) ? (double)partG[(long)c * K + k] : 0.0;
+        int c = 0;
+        for (; c + RU <= NC; c += RU) {
+            float v[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) v[u] = partG[(long)(c + u) * K + k];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) g += live(c + u) ? (double)v[u] : 0.0;
+        }
+        for (; c < NC; ++c) g += live(c) ? (double)partG[(long)c * K + k] : 0.0;
         G[k] = g;
     }
 }

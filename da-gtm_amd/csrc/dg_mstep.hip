@@ -503,8 +503,17 @@ __device__ __forceinline__ void k_sumAB_body(
     const long E = (long)M1 * NCOL;
     const long e = (long)bx * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    // eight loads in flight, then their sums in chunk order
     double s = 0.0;
-    for (int c = 0; c < nch; ++c) s += part[(long)c * E + e];
+    int c = 0;
+    for (; c + 8 <= nch; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(long)(c + u) * E + e];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < nch; ++c) s += part[(long)c * E + e];
     const int i = (int)(e / NCOL), j = (int)(e % NCOL);
     if (j < M1) A0[(long)i * M1 + j] = s;
     else B[(long)i * D + (j - M1)] = s;
