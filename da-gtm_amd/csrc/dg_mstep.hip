@@ -21,8 +21,17 @@ __device__ __forceinline__ void k_rownorm_body(
     long rows, int ld, int D, double *__restrict__ out) {
     const long r = (long)bx * blockDim.x + threadIdx.x;
     if (r >= rows) return;
+    // eight loads in flight, then their FMAs in order
     double s = 0.0;
-    for (int d = 0; d < D; ++d) {
+    int d = 0;
+    for (; d + 8 <= D; d += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = X[r * ld + d + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s = fma(v[u], v[u], s);
+    }
+    for (; d < D; ++d) {
         const double v = X[r * ld + d];
         s = fma(v, v, s);
     }
