@@ -889,14 +889,50 @@ __device__ __forceinline__ void k_beta_part_body(
     __shared__ double s[4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     double acc = 0.0;                              // lane 0: this wave's sum
-    for (int q = 0; q < kBetaWG / 4; ++q) {
-        const int k = bx * kBetaWG + w * (kBetaWG / 4) + q;
-        if (k >= K) break;
-        double dot = 0.0;
-        for (int d = l; d < D; d += 64) dot = fma(Y[(long)k * ld + d], RT[(long)k * D + d], dot);
+    const int kw = bx * kBetaWG + w * (kBetaWG / 4);
+    constexpr int DM = 3;
+    if (D <= 64 * DM) {
+        // the next centre's operands are loaded while this one's dot product
+        // is reduced (same products and sums as the loop below)
+        double yv[DM], rv[DM], gy[2];
+        auto load = [&](int k) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
-        acc += G[k] * yy[k] - 2.0 * dot;
+            for (int m = 0; m < DM; ++m) {
+                const int d = l + 64 * m;
+                const bool in = k < K && d < D;
+                yv[m] = in ? Y[(long)k * ld + d] : 0.0;
+                rv[m] = in ? RT[(long)k * D + d] : 0.0;
+            }
+            gy[0] = (k < K) ? G[k] : 0.0;
+            gy[1] = (k < K) ? yy[k] : 0.0;
+        };
+        load(kw);
+        for (int q = 0; q < kBetaWG / 4; ++q) {
+            const int k = kw + q;
+            if (k >= K) break;
+            double y[DM], r[DM];
+#pragma unroll
+            for (int m = 0; m < DM; ++m) { y[m] = yv[m]; r[m] = rv[m]; }
+            const double gk = gy[0], yk = gy[1];
+            if (q + 1 < kBetaWG / 4) load(k + 1);
+            double dot = 0.0;
+#pragma unroll
+            for (int m = 0; m < DM; ++m)
+                if (l + 64 * m < D) dot = fma(y[m], r[m], dot);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+            acc += gk * yk - 2.0 * dot;
+        }
+    } else {
+        for (int q = 0; q < kBetaWG / 4; ++q) {
+            const int k = kw + q;
+            if (k >= K) break;
+            double dot = 0.0;
+            for (int d = l; d < D; d += 64) dot = fma(Y[(long)k * ld + d], RT[(long)k * D + d], dot);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+            acc += G[k] * yy[k] - 2.0 * dot;
+        }
     }
     if (l == 0) s[w] = acc;
     __syncthreads();
